@@ -1,0 +1,204 @@
+"""MLUPS and HBM-roofline fraction of the fused D3Q19 step on one MI355X.
+
+Workload (the headline configuration): D3Q19 LES (Smagorinsky, Cs 0.16) on 512x256x256 =
+33,554,432 cells, sphere of radius 16 centred at (96, 128, 128), moving top lid u = (0.04, 0, 0)
+over a free-slip floor, inflow u0 = 0.08, Re 3000 (nu = u0 D / Re, D = 32), sponge over the last
+quarter in x, rho = 1 with seeded U(-1e-3, 1e-3) velocity noise, and a report step (drag/lift,
+residuals, FP64 sums) every 100 steps.  A "step" is one lattice update of the whole grid.
+
+value = whole-grid MLUPS over the timed steps (report steps included) with FP32 populations;
+the FP16/BF16 shifted-storage variants are timed the same way and reported under "variants".
+roofline: the quiet FP32 step kernel, 152 algorithmic bytes per cell (19 x 4 B read + 19 x 4 B
+written), over its mean duration from HIP events on the launch stream.
+cpu_baseline: the NumPy FP32 restatement of the same step (the oracle in float32), on a 512x256x4
+slab of the same workload, single-threaded.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--storage f32,f16,bf16]
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tack-lbm_amd"))
+sys.path.insert(0, ROOT)
+
+NX, NY, NZ = 512, 256, 256
+RADIUS = 16.0
+CENTER = (96.0, NY / 2, NZ / 2)
+RE = 3000.0
+U0 = 0.08
+LID = (0.04, 0.0, 0.0)
+REPORT_EVERY = 100
+PEAK_GBS = 8000.0
+BYTES_PER_CELL = {"f32": 152, "f16": 76, "bf16": 76}
+
+
+def make_solver(storage, stream, nx=NX, ny=NY, nz=NZ):
+    import tlbm
+    return tlbm.Solver(nx, ny, nz, collision="les", obstacle="sphere", reynolds=RE, u0=U0,
+                       diameter=2 * RADIUS, lateral="lid", lid_u=LID,
+                       center=(CENTER[0], ny / 2, nz / 2), storage=storage, noise_amp=1e-3,
+                       noise_seed=0, stream=stream)
+
+
+class Events:
+    def __init__(self, lib):
+        self.lib = lib
+
+    def new(self):
+        e = ctypes.c_void_p()
+        from tlbm._capi import check
+        check(self.lib.tlbm_event_create(ctypes.byref(e)), "event create")
+        return e
+
+    def elapsed(self, a, b):
+        ms = ctypes.c_float()
+        from tlbm._capi import check
+        check(self.lib.tlbm_event_elapsed_ms(a, b, ctypes.byref(ms)), "event elapsed")
+        return ms.value
+
+
+def time_gpu(storage, steps, warmup):
+    """Times `steps` steps (report every 100th) and the quiet-kernel mean from HIP events."""
+    import tlbm
+    from tlbm._capi import check, lib
+    L = lib()
+    stream = ctypes.c_void_p()
+    check(L.tlbm_stream_create(ctypes.byref(stream)), "stream create")
+    s = make_solver(storage, stream.value)
+    for _ in range(warmup):
+        s.step(report=(s.step_count % REPORT_EVERY == REPORT_EVERY - 1))
+    ev = Events(L)
+    tlbm.sync(stream.value)
+    check(L.tlbm_device_sync(), "device sync")
+    # quiet runs between report steps are bracketed by events on the launch stream
+    marks = []
+    t0 = time.perf_counter()
+    start = ev.new()
+    check(L.tlbm_event_record(start, stream), "record")
+    run_start, run_len = None, 0
+    for _ in range(steps):
+        report = s.step_count % REPORT_EVERY == REPORT_EVERY - 1
+        if report:
+            if run_len:
+                e = ev.new()
+                check(L.tlbm_event_record(e, stream), "record")
+                marks.append((run_start, e, run_len))
+                run_start, run_len = None, 0
+            s.step(report=True)
+        else:
+            if run_start is None:
+                run_start = ev.new()
+                check(L.tlbm_event_record(run_start, stream), "record")
+            s.step()
+            run_len += 1
+    if run_len:
+        e = ev.new()
+        check(L.tlbm_event_record(e, stream), "record")
+        marks.append((run_start, e, run_len))
+    stop = ev.new()
+    check(L.tlbm_event_record(stop, stream), "record")
+    tlbm.sync(stream.value)
+    check(L.tlbm_device_sync(), "device sync")
+    wall = time.perf_counter() - t0
+    total_ms = ev.elapsed(start, stop)
+    quiet_ms = sum(ev.elapsed(a, b) for a, b, _ in marks)
+    quiet_n = sum(k for _, _, k in marks)
+    m = s.monitors(REPORT_EVERY)
+    n = s.lat.n
+    out = {
+        "storage": storage,
+        "mlups": n * steps / (total_ms * 1e-3) / 1e6,
+        "ms_per_step": total_ms / steps,
+        "wall_ms_per_step": wall * 1e3 / steps,
+        "quiet_kernel_ms": quiet_ms / max(quiet_n, 1),
+        "quiet_launches": quiet_n,
+        "monitors": {k: float(v) for k, v in m.items()},
+    }
+    out["achieved_gbs"] = BYTES_PER_CELL[storage] * n / (out["quiet_kernel_ms"] * 1e-3) / 1e9
+    del s
+    L.tlbm_stream_destroy(stream)
+    return out
+
+
+def cpu_baseline(max_seconds=20.0):
+    """The NumPy float32 restatement on a 512x256x4 slab of the workload, one core."""
+    import numpy as np
+    from oracle import lbm_oracle as O
+    nz = 4
+    s = O.OracleSolver(NX, NY, nz, "les", "sphere", RE, U0, 2 * RADIUS, lateral="lid",
+                       dtype=np.float32, center=(CENTER[0], NY / 2, nz / 2), lid_u=LID,
+                       noise_amp=1e-3, construct_report=False)
+    s.step()  # warm-up (index caches)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        s.step()
+        steps += 1
+        if time.perf_counter() - t0 > max_seconds or steps >= 20:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": s.n * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"NumPy float32 oracle step, {NX}x{NY}x{nz} slab of the workload "
+                      f"(LES, sphere, lid), {steps} steps in {dt:.1f} s; host has "
+                      f"{os.cpu_count()} CPUs, {len(os.sched_getaffinity(0))} usable, NumPy "
+                      "elementwise kernels run on one"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--storage", default="f32,f16,bf16")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is "
+              "described, not built", file=sys.stderr)
+        sys.exit(2)
+    storages = [s for s in args.storage.split(",") if s]
+    results = {}
+    for st in storages:
+        results[st] = time_gpu(st, args.steps, args.warmup)
+        print(f"[bench] {st}: {results[st]['mlups']:.0f} MLUPS, quiet kernel "
+              f"{results[st]['quiet_kernel_ms']:.3f} ms = {results[st]['achieved_gbs']:.0f} GB/s",
+              file=sys.stderr, flush=True)
+    head = results.get("f32") or results[storages[0]]
+    cpu = None if args.no_cpu else cpu_baseline()
+    line = {
+        "metric": "MLUPS & %HBM roofline, D3Q19 LES 512x256x256, FP32 vs FP16-store",
+        "value": round(head["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head["storage"] == "f32" else head["storage"] + "-store/f32-compute",
+        "data": "synthetic",
+        "config": {"workload": "D3Q19 LES 512x256x256, sphere r16 at x=96, lid u=0.04, Re 3000, "
+                               "seeded 1e-3 velocity noise, report every 100 steps",
+                   "cells": NX * NY * NZ, "storage": head["storage"]},
+        "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1),
+                     "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": round(head["achieved_gbs"] / PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "variants": {st: {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+                          "quiet_kernel_ms": round(r["quiet_kernel_ms"], 4),
+                          "achieved_gbs": round(r["achieved_gbs"], 1),
+                          "roofline_frac": round(r["achieved_gbs"] / PEAK_GBS, 4),
+                          "drag": r["monitors"]["drag"], "lift": r["monitors"]["lift"]}
+                     for st, r in results.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+/*
+ * tlbm.h -- C ABI of the MI355X D3Q19 lattice Boltzmann step library (libtlbm.so).
+ *
+ * The reference composes its step from Tack device classes and JIT-compiles one kernel per
+ * combination of plugin types (lbm.py:1-7, README.md:6-9).  This library replaces that boundary
+ * with plain C: a kernel is chosen by a tlbm_spec (collision x obstacle x diagnostics x lateral
+ * side x storage), every combination is a pre-instantiated, branch-free HIP kernel for gfx950,
+ * and the instance scalars the reference passes as kernel arguments travel in tlbm_params.
+ *
+ * Conventions:
+ *   - every function returns TLBM_OK (0) or a nonzero status; tlbm_last_error() then names the
+ *     failure (per host thread).  No C++ exception crosses the ABI.
+ *   - pointers to lattice data are device (or managed) pointers; "stream" is a hipStream_t
+ *     passed as void* (NULL = the null stream).  All launches are asynchronous.
+ *   - layout is the reference's: population q of cell idx = (z*ny + y)*nx + x lives at
+ *     f[q*qstride + idx] (lbm.py:386-388, 426, 456), with qstride >= nx*ny*nz.
+ */
+#ifndef TLBM_H
+#define TLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TLBM_OK 0
+#define TLBM_ERR_INVALID 1      /* bad argument: sizes, null pointers, unknown spec       */
+#define TLBM_ERR_HIP 2          /* a HIP runtime call failed (message has hipGetErrorString) */
+#define TLBM_ERR_NO_DEVICE 3    /* no HIP device visible                                  */
+
+/* collision models: lbm.py:544 COLLISIONS = {"bgk": BGK, "trt": TRT, "les": SmagorinskyLES} */
+enum { TLBM_BGK = 0, TLBM_TRT = 1, TLBM_LES = 2 };
+/* obstacles: lbm.py:144-227 (Cylinder, Sphere, Box) */
+enum { TLBM_CYLINDER = 0, TLBM_SPHERE = 1, TLBM_BOX = 2 };
+/* diagnostics variants: lbm.py:325-339 (Quiet.enabled = 0, Report.enabled = 1) */
+enum { TLBM_QUIET = 0, TLBM_REPORT = 1 };
+/* lateral y sides: lbm.py:342-369 (PeriodicY, SlipY); TLBM_LID_Y has no counterpart in the
+ * reference: a moving no-slip lid halfway outside y = ny-1 over a free-slip floor. */
+enum { TLBM_PERIODIC_Y = 0, TLBM_SLIP_Y = 1, TLBM_LID_Y = 2 };
+/* population storage: FP32 as the reference (lbm.py:405-406, 566); FP16/BF16 store the
+ * deviation f_q - w_q from the rest equilibrium, arithmetic stays FP32. */
+enum { TLBM_F32 = 0, TLBM_F16 = 1, TLBM_BF16 = 2 };
+
+/* Number of FP64 sums a report step produces, in this order:
+ * the 13 fields of lbm.py:545 FIELDS (ux uy uz p vmag r_rho r_ux r_uy r_uz r_ke fx fy fz),
+ * then the total mass sum(rho). */
+#define TLBM_NSUMS 14
+
+typedef struct {
+  int32_t collision; /* TLBM_BGK / TLBM_TRT / TLBM_LES           */
+  int32_t obstacle;  /* TLBM_CYLINDER / TLBM_SPHERE / TLBM_BOX    */
+  int32_t diag;      /* TLBM_QUIET / TLBM_REPORT                  */
+  int32_t side;      /* TLBM_PERIODIC_Y / TLBM_SLIP_Y / TLBM_LID_Y */
+  int32_t storage;   /* TLBM_F32 / TLBM_F16 / TLBM_BF16           */
+} tlbm_spec;
+
+/* Runtime arguments of one step (the reference's plugin instance attributes). */
+typedef struct {
+  int32_t nx, ny, nz;      /* D3Q19(nx, ny, nz), lbm.py:42-44                           */
+  int32_t reserved0;
+  int64_t qstride;         /* elements between population planes, >= nx*ny*nz           */
+  float omega;             /* BGK: 1 / (3 nu + 1/2), lbm.py:63                          */
+  float tau0;              /* TRT / LES: 3 nu + 1/2, lbm.py:110, 131                    */
+  float cs;                /* Smagorinsky constant (TRT.cs, SmagorinskyLES.cs = 0.16)    */
+  float magic;             /* TRT magic parameter (TRT.magic = 0.1875), lbm.py:106       */
+  float obs[6];            /* cylinder: x, y, r; sphere: x, y, z, r;
+                              box: x, y, z, hx, hy, hz  (lbm.py:146, 174, 203-205)       */
+  float u0;                /* Flow.u0, lbm.py:375                                        */
+  float sponge_start;      /* Flow.sponge_start                                          */
+  float sponge_strength;   /* Flow.sponge_strength                                       */
+  float lid[3];            /* TLBM_LID_Y wall velocity                                   */
+  const float* omega_rot;  /* device/managed pointer to one float, the cylinder surface
+                              speed read by every step (Flow.omega_rot, lbm.py:378-379,
+                              417); NULL means 0                                          */
+} tlbm_params;
+
+/* The 13 report fields of lbm.py:545, each n = nx*ny*nz floats (device or managed). */
+typedef struct {
+  float *ux, *uy, *uz, *p, *vmag, *r_rho, *r_ux, *r_uy, *r_uz, *r_ke, *fx, *fy, *fz;
+} tlbm_fields;
+
+/* ---- errors, devices --------------------------------------------------------------------- */
+const char* tlbm_last_error(void);
+int tlbm_version(void);
+int tlbm_device_count(int* count);
+int tlbm_set_device(int device);
+
+/* ---- memory (replaces ManagedArray, lbm.py:525-541, and tack.array/empty/zeros) ------------- */
+/* hipMallocManaged, preferred location = current device, prefetched there, zero-filled. */
+int tlbm_alloc_managed(size_t bytes, void** out);
+int tlbm_alloc_device(size_t bytes, void** out);
+int tlbm_alloc_host(size_t bytes, void** out); /* pinned host memory (hipHostMalloc) */
+int tlbm_free(void* ptr);                      /* device or managed */
+int tlbm_free_host(void* ptr);
+int tlbm_memcpy(void* dst, const void* src, size_t bytes);                   /* synchronous  */
+int tlbm_memcpy_async(void* dst, const void* src, size_t bytes, void* stream);
+int tlbm_memset(void* ptr, int value, size_t bytes, void* stream);          /* asynchronous */
+
+/* ---- streams, events (replaces tack.sync, lbm.py:651) ------------------------------------ */
+int tlbm_stream_create(void** stream);
+int tlbm_stream_destroy(void* stream);
+int tlbm_stream_sync(void* stream);
+int tlbm_device_sync(void);
+int tlbm_event_create(void** event);
+int tlbm_event_destroy(void* event);
+int tlbm_event_record(void* event, void* stream);
+int tlbm_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ---- kernels ----------------------------------------------------------------------------- */
+/* Resolve a specialisation (Tack's type-keyed kernel cache, README.md:55-56).  The handle is
+ * stable for the life of the process. */
+int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle);
+/* Threads per block along x for step launches (64, 128 or 256); 0 restores the default. */
+int tlbm_set_block_x(int handle, int block_x);
+
+/* lbm_init (lbm.py:383-394): f = feq(1, u0 fluid, seed_amp u0 fluid sin(2 pi x/nx)
+ * cos(2 pi z/nz), 0), plus, when noise_amp != 0, a seeded uniform U(-noise_amp, noise_amp)
+ * velocity noise on fluid cells (counter-based splitmix64 of (noise_seed, 3 idx + component)).
+ * Uses the obstacle and storage of the handle; collision/diag/side are ignored. */
+int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, float noise_amp,
+              uint64_t noise_seed, void* stream);
+
+/* lbm_step (lbm.py:397-456, 493-513): one fused pull-stream + bounce-back + collide + sponge
+ * step from f_in to f_out (distinct buffers).  For a TLBM_REPORT handle, `fields` must hold the
+ * 13 report arrays (old ux, uy, uz, p are read, all 13 are written) and, when `sums` is not
+ * NULL, TLBM_NSUMS FP64 sums are written to sums[0..13] (device, managed or pinned host
+ * memory) by the same stream.  `fields` and `sums` are ignored by TLBM_QUIET handles.
+ * Replaces the host FP64 numpy sums of Solver.monitors (lbm.py:649-661). */
+int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* params,
+              const tlbm_fields* fields, double* sums, void* stream);
+
+/* coarsen (lbm.py:516-522): dst[(k*my + j)*mx + i] = src[((k s)*ny + j s)*nx + i s]. */
+int tlbm_coarsen(const float* src, float* dst, int nx, int ny, int nz, int s, int mx, int my,
+                 int mz, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TLBM_H */

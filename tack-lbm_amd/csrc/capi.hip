@@ -1,0 +1,455 @@
+// extern "C" boundary of libtlbm.so (declared in include/tlbm.h).
+//
+// Host side of the step library: argument checks, the specialisation table, launch shapes,
+// the report-step FP64 reduction, managed/device memory, streams and events.  Everything a
+// caller needs to drive the step from C, ctypes, cgo or JNI without C++ or torch types.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "registry.hpp"
+#include "tlbm.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(TLBM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define TLBM_HIP(call)                                      \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);       \
+  } while (0)
+
+// ---- specialisation table ------------------------------------------------------------------
+constexpr int kHandles = 3 * 3 * 3 * 3 * 2;  // storage, collision, obstacle, side, diag
+
+tlbm::StepTable g_tables[3][3];
+int g_block_x[kHandles];
+std::once_flag g_once;
+
+void build_tables() {
+  tlbm_register_0_0(g_tables[0][0]);
+  tlbm_register_0_1(g_tables[0][1]);
+  tlbm_register_0_2(g_tables[0][2]);
+  tlbm_register_1_0(g_tables[1][0]);
+  tlbm_register_1_1(g_tables[1][1]);
+  tlbm_register_1_2(g_tables[1][2]);
+  tlbm_register_2_0(g_tables[2][0]);
+  tlbm_register_2_1(g_tables[2][1]);
+  tlbm_register_2_2(g_tables[2][2]);
+  for (int& b : g_block_x) b = 0;
+}
+
+struct Decoded {
+  int storage, collision, obstacle, side, diag;
+};
+
+bool decode(int h, Decoded& d) {
+  if (h < 0 || h >= kHandles) return false;
+  d.diag = h % 2;
+  h /= 2;
+  d.side = h % 3;
+  h /= 3;
+  d.obstacle = h % 3;
+  h /= 3;
+  d.collision = h % 3;
+  h /= 3;
+  d.storage = h;
+  return true;
+}
+
+// ---- per-device report workspace (grow-only) ----------------------------------------------
+struct Workspace {
+  double* ptr = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+std::vector<Workspace> g_ws;
+
+int workspace(size_t bytes, double** out) {
+  int dev = 0;
+  TLBM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  if (static_cast<int>(g_ws.size()) <= dev) g_ws.resize(dev + 1);
+  Workspace& w = g_ws[dev];
+  if (w.bytes < bytes) {
+    if (w.ptr) {
+      TLBM_HIP(hipDeviceSynchronize());
+      TLBM_HIP(hipFree(w.ptr));
+      w.ptr = nullptr;
+      w.bytes = 0;
+    }
+    void* p = nullptr;
+    TLBM_HIP(hipMalloc(&p, bytes));
+    w.ptr = static_cast<double*>(p);
+    w.bytes = bytes;
+  }
+  *out = w.ptr;
+  return TLBM_OK;
+}
+
+// Sums partials[k][0..nblocks) in a fixed order: deterministic FP64 totals.
+__global__ void __launch_bounds__(1024) reduce_partials(const double* __restrict__ partials,
+                                                        uint32_t nblocks, double* __restrict__ out) {
+  __shared__ double red[1024];
+  for (int k = 0; k < TLBM_NSUMS; ++k) {
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < nblocks; i += blockDim.x)
+      s += partials[static_cast<size_t>(k) * nblocks + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void coarsen_kernel(const float* __restrict__ src, float* __restrict__ dst, int nx,
+                               int ny, int s, int mx, int my, int mz) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t m = static_cast<uint32_t>(mx) * my * mz;
+  if (i >= m) return;
+  const uint32_t a = i % mx, b = (i / mx) % my, c = i / (static_cast<uint32_t>(mx) * my);
+  dst[i] = src[(static_cast<size_t>(c) * s * ny + static_cast<size_t>(b) * s) * nx + a * s];
+}
+
+int check_params(const tlbm_params* p) {
+  if (!p) return fail(TLBM_ERR_INVALID, "params is NULL");
+  if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0)
+    return fail(TLBM_ERR_INVALID, "grid %dx%dx%d must be positive", p->nx, p->ny, p->nz);
+  if (p->ny > 65535 || p->nz > 65535)
+    return fail(TLBM_ERR_INVALID, "ny and nz must be <= 65535 (got %d, %d)", p->ny, p->nz);
+  const uint64_t n = static_cast<uint64_t>(p->nx) * p->ny * p->nz;
+  if (n > 0xFFFFFFFFull)
+    return fail(TLBM_ERR_INVALID, "%llu cells exceed the 32-bit cell index",
+                static_cast<unsigned long long>(n));
+  if (p->qstride < static_cast<int64_t>(n))
+    return fail(TLBM_ERR_INVALID, "qstride %lld < cells %llu", static_cast<long long>(p->qstride),
+                static_cast<unsigned long long>(n));
+  if (static_cast<uint64_t>(p->qstride) * 18 + n > (1ull << 40))
+    return fail(TLBM_ERR_INVALID, "population buffer too large");
+  return TLBM_OK;
+}
+
+tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
+  tlbm::StepArgs a{};
+  a.ns = static_cast<uint64_t>(p.qstride);
+  a.nx = p.nx;
+  a.ny = p.ny;
+  a.nz = p.nz;
+  a.omega = p.omega;
+  a.tau0 = p.tau0;
+  a.smag = 25.456f * p.cs * p.cs;
+  a.magic = p.magic;
+  for (int i = 0; i < 6; ++i) a.o[i] = p.obs[i];
+  float r = 0.0f;
+  if (obstacle == TLBM_CYLINDER) {
+    r = p.obs[2];
+    a.reach[0] = a.reach[1] = std::fabs(r) + 2.0f;
+    a.reach[2] = 0.0f;
+  } else if (obstacle == TLBM_SPHERE) {
+    r = p.obs[3];
+    a.reach[0] = a.reach[1] = a.reach[2] = std::fabs(r) + 2.0f;
+  } else {
+    a.reach[0] = std::fabs(p.obs[3]) + 2.0f;
+    a.reach[1] = std::fabs(p.obs[4]) + 2.0f;
+    a.reach[2] = std::fabs(p.obs[5]) + 2.0f;
+  }
+  a.r2 = r * r;
+  a.u0 = p.u0;
+  a.s0 = p.sponge_start;
+  a.s_len = static_cast<float>(p.nx) - p.sponge_start;
+  a.s_k = p.sponge_strength;
+  for (int i = 0; i < 3; ++i) a.lid[i] = p.lid[i];
+  a.omega_rot = p.omega_rot;
+  return a;
+}
+
+int default_block_x(int nx) {
+  if (nx <= 64) return 64;
+  if (nx <= 128) return 128;
+  return 256;
+}
+
+template <int O, int S>
+hipError_t launch_init(const tlbm::StepArgs& a, float seed_amp, float noise_amp, uint64_t seed,
+                       dim3 g, dim3 b, hipStream_t st) {
+  hipLaunchKernelGGL((tlbm::lbm_init_kernel<O, S>), g, b, 0, st, a, seed_amp, noise_amp, seed);
+  return hipGetLastError();
+}
+
+using InitLaunch = hipError_t (*)(const tlbm::StepArgs&, float, float, uint64_t, dim3, dim3,
+                                  hipStream_t);
+const InitLaunch kInit[3][3] = {
+    {&launch_init<0, 0>, &launch_init<1, 0>, &launch_init<2, 0>},
+    {&launch_init<0, 1>, &launch_init<1, 1>, &launch_init<2, 1>},
+    {&launch_init<0, 2>, &launch_init<1, 2>, &launch_init<2, 2>}};  // [storage][obstacle]
+
+}  // namespace
+
+extern "C" {
+
+const char* tlbm_last_error(void) { return g_error.c_str(); }
+
+int tlbm_version(void) { return 1; }
+
+int tlbm_device_count(int* count) {
+  if (!count) return fail(TLBM_ERR_INVALID, "count is NULL");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(TLBM_ERR_NO_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  }
+  *count = n;
+  return TLBM_OK;
+}
+
+int tlbm_set_device(int device) {
+  TLBM_HIP(hipSetDevice(device));
+  return TLBM_OK;
+}
+
+int tlbm_alloc_managed(size_t bytes, void** out) {
+  if (!out) return fail(TLBM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (bytes == 0) bytes = 4;
+  int dev = 0;
+  TLBM_HIP(hipGetDevice(&dev));
+  void* p = nullptr;
+  hipError_t e = hipMallocManaged(&p, bytes, hipMemAttachGlobal);
+  if (e != hipSuccess) return hip_fail(e, "hipMallocManaged");
+  // Keep the pages on the GPU: the kernels write these fields, host readers (in-situ I/O)
+  // fault them over on demand.  Advice is a hint; a runtime that declines it is not an error.
+  if (hipMemAdvise(p, bytes, hipMemAdviseSetPreferredLocation, dev) != hipSuccess)
+    (void)hipGetLastError();
+  if (hipMemPrefetchAsync(p, bytes, dev, nullptr) != hipSuccess) (void)hipGetLastError();
+  e = hipMemset(p, 0, bytes);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    return hip_fail(e, "zero-fill of managed allocation");
+  }
+  *out = p;
+  return TLBM_OK;
+}
+
+int tlbm_alloc_device(size_t bytes, void** out) {
+  if (!out) return fail(TLBM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (bytes == 0) bytes = 4;
+  TLBM_HIP(hipMalloc(out, bytes));
+  return TLBM_OK;
+}
+
+int tlbm_alloc_host(size_t bytes, void** out) {
+  if (!out) return fail(TLBM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (bytes == 0) bytes = 4;
+  TLBM_HIP(hipHostMalloc(out, bytes, hipHostMallocDefault));
+  return TLBM_OK;
+}
+
+int tlbm_free(void* ptr) {
+  if (ptr) TLBM_HIP(hipFree(ptr));
+  return TLBM_OK;
+}
+
+int tlbm_free_host(void* ptr) {
+  if (ptr) TLBM_HIP(hipHostFree(ptr));
+  return TLBM_OK;
+}
+
+int tlbm_memcpy(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return TLBM_OK;
+  if (!dst || !src) return fail(TLBM_ERR_INVALID, "memcpy with NULL pointer");
+  TLBM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+  return TLBM_OK;
+}
+
+int tlbm_memcpy_async(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return TLBM_OK;
+  if (!dst || !src) return fail(TLBM_ERR_INVALID, "memcpy with NULL pointer");
+  TLBM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, static_cast<hipStream_t>(stream)));
+  return TLBM_OK;
+}
+
+int tlbm_memset(void* ptr, int value, size_t bytes, void* stream) {
+  if (bytes == 0) return TLBM_OK;
+  if (!ptr) return fail(TLBM_ERR_INVALID, "memset of NULL");
+  TLBM_HIP(hipMemsetAsync(ptr, value, bytes, static_cast<hipStream_t>(stream)));
+  return TLBM_OK;
+}
+
+int tlbm_stream_create(void** stream) {
+  if (!stream) return fail(TLBM_ERR_INVALID, "stream is NULL");
+  hipStream_t s;
+  TLBM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return TLBM_OK;
+}
+
+int tlbm_stream_destroy(void* stream) {
+  if (stream) TLBM_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return TLBM_OK;
+}
+
+int tlbm_stream_sync(void* stream) {
+  TLBM_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return TLBM_OK;
+}
+
+int tlbm_device_sync(void) {
+  TLBM_HIP(hipDeviceSynchronize());
+  return TLBM_OK;
+}
+
+int tlbm_event_create(void** event) {
+  if (!event) return fail(TLBM_ERR_INVALID, "event is NULL");
+  hipEvent_t e;
+  TLBM_HIP(hipEventCreate(&e));
+  *event = e;
+  return TLBM_OK;
+}
+
+int tlbm_event_destroy(void* event) {
+  if (event) TLBM_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return TLBM_OK;
+}
+
+int tlbm_event_record(void* event, void* stream) {
+  if (!event) return fail(TLBM_ERR_INVALID, "event is NULL");
+  TLBM_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+  return TLBM_OK;
+}
+
+int tlbm_event_elapsed_ms(void* start, void* stop, float* ms) {
+  if (!start || !stop || !ms) return fail(TLBM_ERR_INVALID, "NULL event or output");
+  TLBM_HIP(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+  TLBM_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+  return TLBM_OK;
+}
+
+int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle) {
+  if (!spec || !handle) return fail(TLBM_ERR_INVALID, "spec or handle is NULL");
+  std::call_once(g_once, build_tables);
+  const tlbm_spec& s = *spec;
+  if (s.collision < 0 || s.collision > 2) return fail(TLBM_ERR_INVALID, "collision %d", s.collision);
+  if (s.obstacle < 0 || s.obstacle > 2) return fail(TLBM_ERR_INVALID, "obstacle %d", s.obstacle);
+  if (s.diag < 0 || s.diag > 1) return fail(TLBM_ERR_INVALID, "diag %d", s.diag);
+  if (s.side < 0 || s.side > 2) return fail(TLBM_ERR_INVALID, "side %d", s.side);
+  if (s.storage < 0 || s.storage > 2) return fail(TLBM_ERR_INVALID, "storage %d", s.storage);
+  *handle = ((((s.storage * 3 + s.collision) * 3 + s.obstacle) * 3 + s.side) * 2) + s.diag;
+  return TLBM_OK;
+}
+
+int tlbm_set_block_x(int handle, int block_x) {
+  std::call_once(g_once, build_tables);
+  Decoded d;
+  if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
+  if (block_x != 0 && block_x != 64 && block_x != 128 && block_x != 256)
+    return fail(TLBM_ERR_INVALID, "block_x must be 64, 128 or 256 (got %d)", block_x);
+  g_block_x[handle] = block_x;
+  return TLBM_OK;
+}
+
+int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, float noise_amp,
+              uint64_t noise_seed, void* stream) {
+  std::call_once(g_once, build_tables);
+  Decoded d;
+  if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
+  if (!f) return fail(TLBM_ERR_INVALID, "f is NULL");
+  if (int rc = check_params(params)) return rc;
+  tlbm::StepArgs a = make_args(*params, d.obstacle);
+  a.fout = f;
+  const dim3 block(256);
+  const dim3 grid((params->nx + 255) / 256, params->ny, params->nz);
+  hipError_t e = kInit[d.storage][d.obstacle](a, seed_amp, noise_amp, noise_seed, grid, block,
+                                              static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "lbm_init launch");
+  return TLBM_OK;
+}
+
+int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* params,
+              const tlbm_fields* fields, double* sums, void* stream) {
+  std::call_once(g_once, build_tables);
+  Decoded d;
+  if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
+  if (!f_in || !f_out) return fail(TLBM_ERR_INVALID, "f_in or f_out is NULL");
+  if (f_in == f_out) return fail(TLBM_ERR_INVALID, "f_in and f_out must be distinct buffers");
+  if (int rc = check_params(params)) return rc;
+  tlbm::StepArgs a = make_args(*params, d.obstacle);
+  a.fin = f_in;
+  a.fout = f_out;
+  const int bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(params->nx);
+  const dim3 block(bx);
+  const dim3 grid((params->nx + bx - 1) / bx, params->ny, params->nz);
+  const uint32_t nblocks = grid.x * grid.y * grid.z;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (d.diag == TLBM_REPORT) {
+    if (!fields) return fail(TLBM_ERR_INVALID, "a report step needs the 13 field arrays");
+    const float* const* ptrs = reinterpret_cast<const float* const*>(fields);
+    for (int k = 0; k < 13; ++k)
+      if (!ptrs[k]) return fail(TLBM_ERR_INVALID, "report field %d is NULL", k);
+    a.fld = *fields;
+    if (sums) {
+      double* ws = nullptr;
+      if (int rc = workspace(sizeof(double) * TLBM_NSUMS * nblocks, &ws)) return rc;
+      a.partials = ws;
+      a.nblocks = nblocks;
+    }
+  }
+  hipError_t e = g_tables[d.storage][d.collision][d.obstacle][d.side][d.diag](a, grid, block, st);
+  if (e != hipSuccess) return hip_fail(e, "lbm_step launch");
+  if (d.diag == TLBM_REPORT && sums) {
+    hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(1024), 0, st, a.partials, nblocks, sums);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "report reduction launch");
+  }
+  return TLBM_OK;
+}
+
+int tlbm_coarsen(const float* src, float* dst, int nx, int ny, int nz, int s, int mx, int my,
+                 int mz, void* stream) {
+  if (!src || !dst) return fail(TLBM_ERR_INVALID, "src or dst is NULL");
+  if (nx <= 0 || ny <= 0 || nz <= 0 || s <= 0 || mx <= 0 || my <= 0 || mz <= 0)
+    return fail(TLBM_ERR_INVALID, "coarsen sizes must be positive");
+  if (static_cast<int64_t>(mx - 1) * s >= nx || static_cast<int64_t>(my - 1) * s >= ny ||
+      static_cast<int64_t>(mz - 1) * s >= nz)
+    return fail(TLBM_ERR_INVALID, "coarse grid %dx%dx%d stride %d exceeds %dx%dx%d", mx, my, mz,
+                s, nx, ny, nz);
+  const uint64_t m = static_cast<uint64_t>(mx) * my * mz;
+  if (m > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "coarse grid too large");
+  const uint32_t blocks = static_cast<uint32_t>((m + 255) / 256);
+  hipLaunchKernelGGL(coarsen_kernel, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     src, dst, nx, ny, s, mx, my, mz);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "coarsen launch");
+  return TLBM_OK;
+}
+
+}  // extern "C"

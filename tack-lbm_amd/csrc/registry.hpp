@@ -1,0 +1,44 @@
+// Kernel registry shared by the instantiation units and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lbm_step.hpp"
+
+namespace tlbm {
+
+using StepLaunch = hipError_t (*)(const StepArgs&, dim3 grid, dim3 block, hipStream_t stream);
+// Launchers of one (storage, collision) unit, indexed [obstacle][side][report].
+using StepTable = StepLaunch[3][3][2];
+
+template <int C, int O, int S, bool R, int ST>
+hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t stream) {
+  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST>), grid, block, 0, stream, a);
+  return hipGetLastError();
+}
+
+template <int C, int ST>
+void fill_table(StepTable& t) {
+  static_for<0, 3>([&](auto o) {
+    static_for<0, 3>([&](auto s) {
+      constexpr int O = decltype(o)::value, S = decltype(s)::value;
+      t[O][S][0] = &launch_step<C, O, S, false, ST>;
+      t[O][S][1] = &launch_step<C, O, S, true, ST>;
+    });
+  });
+}
+
+}  // namespace tlbm
+
+// One per (storage, collision) unit; defined in step_inst.hip compiled with
+// -DTLBM_STORAGE=<s> -DTLBM_COLL=<c>.
+#define TLBM_DECLARE_UNIT(s, c) void tlbm_register_##s##_##c(tlbm::StepTable& t);
+TLBM_DECLARE_UNIT(0, 0)
+TLBM_DECLARE_UNIT(0, 1)
+TLBM_DECLARE_UNIT(0, 2)
+TLBM_DECLARE_UNIT(1, 0)
+TLBM_DECLARE_UNIT(1, 1)
+TLBM_DECLARE_UNIT(1, 2)
+TLBM_DECLARE_UNIT(2, 0)
+TLBM_DECLARE_UNIT(2, 1)
+TLBM_DECLARE_UNIT(2, 2)

@@ -1,0 +1,12 @@
+"""tlbm: an MI355X-native D3Q19 lattice Boltzmann step behind the reference's solver API.
+
+``tlbm.lbm`` mirrors the reference module ``lbm.py`` (plugins, ``lbm_init``, ``lbm_step``,
+``coarsen``, ``ManagedArray``, ``Solver``); the compute runs in ``libtlbm.so`` (hand-written
+HIP for gfx950, C ABI in include/tlbm.h).
+"""
+
+from . import _capi  # noqa: F401
+from .lbm import (BGK, COLLISIONS, CX, CY, CZ, FIELDS, TRT, W, Box, Cylinder, D3Q19,  # noqa: F401
+                  DeviceArray, Flow, HostBuffer, LidY, ManagedArray, NoHeat, PeriodicY, Quiet,
+                  Report, SlipY, SmagorinskyLES, Solver, Sphere, array, coarsen,
+                  decode_populations, empty, encode_populations, lbm_init, lbm_step, sync, zeros)

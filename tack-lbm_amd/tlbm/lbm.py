@@ -1,0 +1,591 @@
+"""Host mirror of the reference's solver API, driving the HIP step library through ctypes.
+
+Same names, argument meanings and layout as the reference module (lbm.py): lattice, collision,
+obstacle, diagnostics, side and flow "plugins" are plain Python objects whose *types* pick one
+pre-compiled kernel specialisation (the reference's Tack compiles one per type combination,
+lbm.py:1-7, README.md:55-56) and whose instance attributes become the kernel's runtime
+arguments.  ``lbm_step`` / ``lbm_init`` / ``coarsen`` keep the reference's kernel signatures;
+``Solver`` keeps its constructor, ``step``, ``monitors`` and field layout (lbm.py:548-661).
+
+Differences, by design:
+  * the step runs on libtlbm.so (hand-written HIP for gfx950); there is no CPU path;
+  * ``Solver.monitors`` reads FP64 sums reduced inside the report kernel instead of summing the
+    13 managed fields on the host (lbm.py:654); the fields are still written, so in-situ
+    readers see the same zero-copy arrays;
+  * population storage can be FP16 or BF16 deviations f_q - w_q (``storage="f16"/"bf16"``);
+  * ``LidY`` adds a moving lid; obstacle centre, sponge start, Cs/magic and a seeded velocity
+    noise can be overridden;
+  * the D3Q7 heat variant (lbm.py:279-319, 457-492) is not built: ``heat=True`` raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib
+
+# ---- lattice tables (lbm.py:15-35) ---------------------------------------------------------------
+CX = [0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0]
+CY = [0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1]
+CZ = [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1]
+W = [1 / 3] + [1 / 18] * 6 + [1 / 36] * 12
+
+
+def _opposite():
+    return [next(p for p in range(19) if (CX[p], CY[p], CZ[p]) == (-CX[q], -CY[q], -CZ[q]))
+            for q in range(19)]
+
+
+def _mirror_y():
+    return [next(p for p in range(19) if (CX[p], CY[p], CZ[p]) == (CX[q], -CY[q], CZ[q]))
+            for q in range(19)]
+
+
+W32 = np.array(W, np.float32)
+
+STORAGES = {"f32": _capi.F32, "f16": _capi.F16, "bf16": _capi.BF16}
+_RAW_DTYPE = {"f32": np.float32, "f16": np.float16, "bf16": np.uint16}
+
+
+# ---- device memory (tack.array / tack.empty / tack.zeros, lbm.py:45-50, 566) ---------------------
+class DeviceArray:
+    """A flat device allocation of ``size`` elements in one of the storage formats.
+
+    ``f32`` holds float32 values; ``f16``/``bf16`` population buffers hold deviations
+    f_q - w_q in binary16 / bfloat16 (raw uint16 on the host side for bf16).
+    """
+
+    def __init__(self, size, storage="f32"):
+        if storage not in STORAGES:
+            raise ValueError(f"storage must be one of {sorted(STORAGES)}, got {storage!r}")
+        self.size = int(size)
+        self.storage = storage
+        self.dtype = np.dtype(_RAW_DTYPE[storage])
+        self.nbytes = self.size * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().tlbm_alloc_device(self.nbytes, ctypes.byref(p)), "device allocation")
+        self.ptr = p.value
+
+    def numpy(self):
+        """A host copy of the raw contents."""
+        out = np.empty(self.size, self.dtype)
+        check(lib().tlbm_memcpy(out.ctypes.data, self.ptr, self.nbytes), "device-to-host copy")
+        return out
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype).reshape(-1)
+        if a.size != self.size:
+            raise ValueError(f"size {a.size} != {self.size}")
+        check(lib().tlbm_memcpy(self.ptr, a.ctypes.data, self.nbytes), "host-to-device copy")
+        return self
+
+    def zero_(self, stream=None):
+        check(lib().tlbm_memset(self.ptr, 0, self.nbytes, stream), "memset")
+        return self
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().tlbm_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def array(a, storage="f32"):
+    """tack.array: upload a NumPy array (raw storage values)."""
+    a = np.asarray(a)
+    return DeviceArray(a.size, storage).copy_from(a)
+
+
+def empty(shape, storage="f32"):
+    size = int(np.prod(shape))
+    return DeviceArray(size, storage)
+
+
+def zeros(shape, storage="f32"):
+    return empty(shape, storage).zero_()
+
+
+def encode_populations(f, storage, q_axis=0):
+    """Float populations (19 along ``q_axis``) -> raw storage values (f32, or f - w_q for
+    f16/bf16 with round-to-nearest-even), the format the step kernels read and write."""
+    f = np.asarray(f, np.float32)
+    if storage == "f32":
+        return f.copy()
+    shape = [1] * f.ndim
+    shape[q_axis] = 19
+    dev = (f - W32.reshape(shape)).astype(np.float32)
+    if storage == "f16":
+        return dev.astype(np.float16)
+    u = dev.view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def decode_populations(raw, storage, q_axis=0):
+    """Raw storage values -> float32 populations (inverse of ``encode_populations``)."""
+    raw = np.asarray(raw)
+    if storage == "f32":
+        return raw.astype(np.float32)
+    shape = [1] * raw.ndim
+    shape[q_axis] = 19
+    if storage == "f16":
+        dev = raw.astype(np.float32)
+    else:
+        dev = (raw.astype(np.uint32) << 16).view(np.float32)
+    return (W32.reshape(shape) + dev).astype(np.float32)
+
+
+class ManagedArray:
+    """A managed allocation seen by the kernels and, zero-copy, as a NumPy array (lbm.py:525-541).
+
+    The pages prefer the GPU; host reads fault them over, so read ``numpy`` only after the
+    stream that wrote it has been synchronised.
+    """
+
+    def __init__(self, n, dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        self.size = int(n)
+        p = ctypes.c_void_p()
+        check(lib().tlbm_alloc_managed(self.size * self.dtype.itemsize, ctypes.byref(p)),
+              "hipMallocManaged")
+        self.ptr = p.value
+        ctype = {np.dtype(np.float32): ctypes.c_float, np.dtype(np.float64): ctypes.c_double}[
+            self.dtype]
+        self.numpy = np.ctypeslib.as_array((ctype * self.size).from_address(self.ptr))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self.numpy = None
+            lib().tlbm_free(self.ptr)
+            self.ptr = None
+
+
+class HostBuffer:
+    """Pinned host memory the GPU can write directly (used for the report-step FP64 sums)."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.size = int(n)
+        p = ctypes.c_void_p()
+        check(lib().tlbm_alloc_host(self.size * self.dtype.itemsize, ctypes.byref(p)),
+              "hipHostMalloc")
+        self.ptr = p.value
+        ctype = {np.dtype(np.float64): ctypes.c_double, np.dtype(np.float32): ctypes.c_float}[
+            self.dtype]
+        self.numpy = np.ctypeslib.as_array((ctype * self.size).from_address(self.ptr))
+        self.numpy[:] = 0
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self.numpy = None
+            lib().tlbm_free_host(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ptr
+
+
+def sync(stream=None):
+    """tack.sync (lbm.py:651): wait for the stream (None: the whole device)."""
+    if stream is None:
+        check(lib().tlbm_device_sync(), "device synchronize")
+    else:
+        check(lib().tlbm_stream_sync(stream), "stream synchronize")
+
+
+# ---- lattice and physics plugins (lbm.py:38-380) ---------------------------------------------------
+class D3Q19:
+    """lbm.py:38-55.  The velocity tables are compiled into the kernels."""
+    Q = 19
+
+    def __init__(self, nx, ny, nz):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.n = self.nx * self.ny * self.nz
+        self.cx = np.array(CX, np.int32)
+        self.cy = np.array(CY, np.int32)
+        self.cz = np.array(CZ, np.int32)
+        self.w = np.array(W, np.float32)
+        self.opp = np.array(_opposite(), np.int32)
+        self.miry = np.array(_mirror_y(), np.int32)
+
+
+class BGK:
+    """lbm.py:60-69."""
+    code = _capi.BGK
+
+    def __init__(self, nu):
+        self.omega = 1.0 / (3.0 * nu + 0.5)
+
+
+class TRT:
+    """lbm.py:96-123: TRT with a Smagorinsky subgrid viscosity; ``magic`` and ``cs`` are class
+    constants as in the reference and may be overridden per instance."""
+    code = _capi.TRT
+    magic = 0.1875
+    cs = 0.16
+
+    def __init__(self, nu, magic=None, cs=None):
+        self.tau0 = 3.0 * nu + 0.5
+        if magic is not None:
+            self.magic = magic
+        if cs is not None:
+            self.cs = cs
+
+
+class SmagorinskyLES:
+    """lbm.py:126-139."""
+    code = _capi.LES
+    cs = 0.16
+
+    def __init__(self, nu, cs=None):
+        self.tau0 = 3.0 * nu + 0.5
+        if cs is not None:
+            self.cs = cs
+
+
+class Cylinder:
+    """lbm.py:144-169 (axis z)."""
+    code = _capi.CYLINDER
+
+    def __init__(self, x, y, r):
+        self.x, self.y, self.r = x, y, r
+
+    def obs(self):
+        return (self.x, self.y, self.r, 0.0, 0.0, 0.0)
+
+
+class Sphere:
+    """lbm.py:172-198."""
+    code = _capi.SPHERE
+
+    def __init__(self, x, y, z, r):
+        self.x, self.y, self.z, self.r = x, y, z, r
+
+    def obs(self):
+        return (self.x, self.y, self.z, self.r, 0.0, 0.0)
+
+
+class Box:
+    """lbm.py:201-227."""
+    code = _capi.BOX
+
+    def __init__(self, x, y, z, hx, hy, hz):
+        self.x, self.y, self.z = x, y, z
+        self.hx, self.hy, self.hz = hx, hy, hz
+
+    def obs(self):
+        return (self.x, self.y, self.z, self.hx, self.hy, self.hz)
+
+
+class NoHeat:
+    """lbm.py:282-308: the thermal variant compiled out (``enabled = 0``)."""
+    enabled = 0
+
+
+class Quiet:
+    """lbm.py:325-334: diagnostics compiled out."""
+    enabled = 0
+
+    def __init__(self, fields):
+        self.fields = fields
+        for name in _capi.FIELD_NAMES:
+            setattr(self, name, fields[name])
+
+
+class Report(Quiet):
+    """lbm.py:337-339: moments, residuals and forces compiled in."""
+    enabled = 1
+
+
+class PeriodicY:
+    """lbm.py:342-352."""
+    code = _capi.PERIODIC_Y
+
+
+class SlipY:
+    """lbm.py:355-369."""
+    code = _capi.SLIP_Y
+
+
+class LidY:
+    """A moving no-slip lid halfway outside y = ny - 1 over a free-slip floor (no counterpart in
+    the reference).  Ceiling links take f_q = f_opp + 6 w_q c_q . u."""
+    code = _capi.LID_Y
+
+    def __init__(self, ux=0.04, uy=0.0, uz=0.0):
+        self.u = (float(ux), float(uy), float(uz))
+
+
+class Flow:
+    """lbm.py:372-380.  ``omega_rot`` is a one-element device/managed array so that host writes
+    reach the kernel."""
+
+    def __init__(self, u0, sponge_start, sponge_strength, omega_rot=None, side=None):
+        self.u0 = u0
+        self.sponge_start = sponge_start
+        self.sponge_strength = sponge_strength
+        self.omega_rot = zeros((1,)) if omega_rot is None else omega_rot
+        self.side = PeriodicY() if side is None else side
+
+
+# ---- kernels (lbm.py:383-522) ----------------------------------------------------------------------
+_HANDLES = {}
+
+
+def kernel_handle(collision, obstacle, diag, side, storage):
+    """The specialisation for this combination of plugin types (cached like Tack's cache)."""
+    key = (collision, obstacle, diag, side, storage)
+    h = _HANDLES.get(key)
+    if h is None:
+        spec = _capi.Spec(collision, obstacle, diag, side, storage)
+        out = ctypes.c_int(-1)
+        check(lib().tlbm_kernel_lookup(ctypes.byref(spec), ctypes.byref(out)), "kernel lookup")
+        h = _HANDLES[key] = out.value
+    return h
+
+
+def _code(obj, what):
+    code = getattr(type(obj), "code", None)
+    if code is None:
+        raise TypeError(f"{type(obj).__name__} is not a known {what} plugin")
+    return code
+
+
+def make_params(lat, phys, geo, flow, qstride=None):
+    """The runtime arguments of a step: instance attributes of the plugins."""
+    p = _capi.Params()
+    p.nx, p.ny, p.nz = lat.nx, lat.ny, lat.nz
+    p.qstride = lat.n if qstride is None else int(qstride)
+    if phys is not None:
+        p.omega = getattr(phys, "omega", 0.0)
+        p.tau0 = getattr(phys, "tau0", 0.0)
+        p.cs = getattr(phys, "cs", 0.0)
+        p.magic = getattr(phys, "magic", 0.0)
+    for i, v in enumerate(geo.obs()):
+        p.obs[i] = v
+    p.u0 = flow.u0
+    p.sponge_start = flow.sponge_start
+    p.sponge_strength = flow.sponge_strength
+    lid = getattr(flow.side, "u", (0.0, 0.0, 0.0))
+    for i in range(3):
+        p.lid[i] = lid[i]
+    p.omega_rot = _ptr(flow.omega_rot)
+    return p
+
+
+def _qstride(lat, f):
+    if f.size % 19:
+        raise ValueError(f"population buffer of {f.size} elements is not 19 planes")
+    qs = f.size // 19
+    if qs < lat.n:
+        raise ValueError(f"population buffer holds {f.size} < 19 * {lat.n} elements")
+    return qs
+
+
+def lbm_init(lat, geo, flow, f, seed_amp, noise_amp=0.0, noise_seed=0, stream=None):
+    """lbm.py:383-394, plus the optional seeded velocity noise U(-noise_amp, noise_amp)."""
+    h = kernel_handle(_capi.BGK, _code(geo, "obstacle"), _capi.QUIET, _capi.PERIODIC_Y,
+                      STORAGES[f.storage])
+    p = make_params(lat, None, geo, flow, _qstride(lat, f))
+    check(lib().tlbm_init(h, f.ptr, ctypes.byref(p), seed_amp, noise_amp, noise_seed, stream),
+          "lbm_init")
+
+
+def _fields_struct(fields):
+    fs = _capi.Fields()
+    for name in _capi.FIELD_NAMES:
+        setattr(fs, name, fields[name].ptr)
+    return fs
+
+
+def lbm_step(lat, phys, geo, flow, diag, f_in, f_out, heat=None, g_in=None, g_out=None,
+             sums=None, stream=None):
+    """lbm.py:397-513: one step from ``f_in`` to ``f_out``.
+
+    ``diag`` is ``Quiet``/``Report``; a ``Report`` step writes its 13 fields and, when ``sums``
+    (a ``HostBuffer``/``ManagedArray`` of 14 float64) is given, their FP64 totals plus the total
+    mass.  ``heat`` must be absent or disabled (``enabled == 0``).
+    """
+    if heat is not None and getattr(heat, "enabled", 0) != 0:
+        raise NotImplementedError("the D3Q7 heat variant (lbm.py:457-492) is not built")
+    if f_in.storage != f_out.storage:
+        raise ValueError(f"f_in is {f_in.storage}, f_out is {f_out.storage}")
+    report = getattr(diag, "enabled", 0) == 1
+    h = kernel_handle(_code(phys, "collision"), _code(geo, "obstacle"),
+                      _capi.REPORT if report else _capi.QUIET, _code(flow.side, "side"),
+                      STORAGES[f_in.storage])
+    qs = _qstride(lat, f_in)
+    if _qstride(lat, f_out) != qs:
+        raise ValueError("f_in and f_out have different plane strides")
+    p = make_params(lat, phys, geo, flow, qs)
+    fs = _fields_struct(diag.fields) if report else None
+    check(lib().tlbm_step(h, f_in.ptr, f_out.ptr, ctypes.byref(p),
+                          ctypes.byref(fs) if fs is not None else None,
+                          _ptr(sums) if report else None, stream), "lbm_step")
+
+
+def coarsen(lat, s, src, dst, mx, my, mz, stream=None):
+    """lbm.py:516-522."""
+    check(lib().tlbm_coarsen(src.ptr, dst.ptr, lat.nx, lat.ny, lat.nz, s, mx, my, mz, stream),
+          "coarsen")
+
+
+COLLISIONS = {"bgk": BGK, "trt": TRT, "les": SmagorinskyLES}
+FIELDS = ("ux", "uy", "uz", "p", "vmag", "r_rho", "r_ux", "r_uy", "r_uz", "r_ke", "fx", "fy", "fz")
+SIDES = {"periodic": PeriodicY, "slip": SlipY, "lid": LidY}
+
+
+def _select_device(device):
+    if device in ("cuda", "hip", None):
+        index = None
+    elif isinstance(device, str) and device.split(":")[0] in ("cuda", "hip") and ":" in device:
+        index = int(device.split(":")[1])
+    else:
+        raise ValueError(f"device must be 'cuda' or 'cuda:N', got {device!r}")
+    if _capi.device_count() < 1:
+        raise RuntimeError("no HIP device is visible")
+    if index is not None:
+        check(lib().tlbm_set_device(index), "set device")
+
+
+class Solver:
+    """lbm.py:548-661: owns the two population buffers, the 13 managed fields and the plugins."""
+    OUTPUT = ("ux", "uy", "uz", "p", "vmag")
+
+    def __init__(self, nx=384, ny=128, nz=64, collision="les", obstacle="cylinder",
+                 reynolds=300.0, u0=0.08, diameter=24.0, device="cuda", stride=1,
+                 heat=False, prandtl=0.7, lateral="periodic", *, storage="f32",
+                 lid_u=(0.04, 0.0, 0.0), center=None, sponge_start=None, sponge_strength=1.0,
+                 cs=None, magic=None, seed_amp=0.05, noise_amp=0.0, noise_seed=0,
+                 qstride=None, stream=None):
+        if heat:
+            raise NotImplementedError("the D3Q7 heat variant (lbm.py:279-319) is not built")
+        _select_device(device)
+        self.stream = stream
+        self.lateral = lateral
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.lat = D3Q19(nx, ny, nz)
+        self.u0, self.diameter = u0, diameter
+        self.storage = storage
+        self.step_count = 0
+        self._omega_state = ManagedArray(1)
+        side = LidY(*lid_u) if lateral == "lid" else SIDES[lateral]()
+        self.flow = Flow(u0, 0.75 * nx if sponge_start is None else sponge_start,
+                         sponge_strength, self._omega_state, side)
+        self._center = center
+        self._cs, self._magic = cs, magic
+        self.set_obstacle(obstacle)
+        self.set_physics(collision, reynolds)
+        n = self.lat.n
+        self.qstride = n if qstride is None else int(qstride)
+        if self.qstride < n:
+            raise ValueError(f"qstride {self.qstride} < cells {n}")
+        self.f = [empty((19 * self.qstride,), storage), empty((19 * self.qstride,), storage)]
+        self.fields = {name: ManagedArray(n) for name in FIELDS}
+        self.quiet = Quiet(self.fields)
+        self.report = Report(self.fields)
+        self.sums = HostBuffer(_capi.TLBM_NSUMS)
+        self.heat_enabled = False
+        self.heat = NoHeat()
+        self.prandtl = prandtl
+        self.stride = stride
+        self.out_dims = tuple((d + stride - 1) // stride for d in (nx, ny, nz))
+        if stride > 1:
+            m = self.out_dims[0] * self.out_dims[1] * self.out_dims[2]
+            self.output = {name: ManagedArray(m) for name in self.OUTPUT}
+        else:
+            self.output = {name: self.fields[name] for name in self.OUTPUT}
+        lbm_init(self.lat, self.geo, self.flow, self.f[0], seed_amp, noise_amp, noise_seed,
+                 self.stream)
+        self.step(report=True)
+        self.step_count = 0
+
+    def update_output(self):
+        """lbm.py:606-611."""
+        if self.stride > 1:
+            mx, my, mz = self.out_dims
+            for name in self.output:
+                coarsen(self.lat, self.stride, self.fields[name], self.output[name], mx, my, mz,
+                        self.stream)
+
+    def set_physics(self, collision, reynolds):
+        """lbm.py:613-616: nu = u0 D / Re; a collision change selects another kernel."""
+        self.collision_name, self.reynolds = collision, float(reynolds)
+        self.nu = self.u0 * self.diameter / self.reynolds
+        kw = {}
+        if collision in ("trt", "les") and self._cs is not None:
+            kw["cs"] = self._cs
+        if collision == "trt" and self._magic is not None:
+            kw["magic"] = self._magic
+        self.phys = COLLISIONS[collision](self.nu, **kw)
+
+    def set_actuation(self, omega_rot):
+        """lbm.py:618-620: cylinder surface speed in lattice units."""
+        self._omega_state.numpy[0] = float(omega_rot)
+
+    @property
+    def omega_rot(self):
+        return float(self._omega_state.numpy[0])
+
+    def set_obstacle(self, kind):
+        """lbm.py:628-637 (``heatsink`` is part of the heat variant and not built)."""
+        nx, ny, nz, r = self.nx, self.ny, self.nz, self.diameter / 2
+        if self._center is None:
+            x0, y0, z0 = 0.22 * nx, 0.5 * ny + 1.7, 0.5 * nz
+        else:
+            x0, y0, z0 = self._center
+        self.obstacle_name = kind
+        makers = {"cylinder": lambda: Cylinder(x0, y0, r),
+                  "sphere": lambda: Sphere(x0, y0, z0, r),
+                  "box": lambda: Box(x0, y0, z0, r, r, 0.3 * nz)}
+        if kind not in makers:
+            raise KeyError(f"obstacle {kind!r} is not built (have {sorted(makers)})")
+        self.geo = makers[kind]()
+
+    def step(self, report=False):
+        """lbm.py:639-647: one step, then swap the ping-pong buffers."""
+        diag = self.report if report else self.quiet
+        lbm_step(self.lat, self.phys, self.geo, self.flow, diag, self.f[0], self.f[1],
+                 sums=self.sums if report else None, stream=self.stream)
+        self.f.reverse()
+        self.step_count += 1
+
+    def monitors(self, interval):
+        """lbm.py:649-661, from the FP64 sums of the last report step."""
+        self.update_output()
+        sync(self.stream)
+        n = float(self.lat.n)
+        s = self.sums.numpy
+        total = {key: float(s[i]) for i, key in enumerate(FIELDS)}
+        res = {name: float(np.sqrt(total[key] / n)) / max(interval, 1)
+               for name, key in (("continuity", "r_rho"), ("momentum_x", "r_ux"),
+                                 ("momentum_y", "r_uy"), ("momentum_z", "r_uz"),
+                                 ("energy", "r_ke"))}
+        scale = 0.5 * self.u0 * self.u0 * self.diameter * self.nz
+        res["drag"] = total["fx"] / scale
+        res["lift"] = total["fy"] / scale
+        res["mass"] = float(s[13])
+        return res
+
+    def populations(self):
+        """The current populations as float32, shaped (19, n) (unshifted for f16/bf16)."""
+        sync(self.stream)
+        raw = self.f[0].numpy().reshape(19, self.qstride)[:, :self.lat.n]
+        return decode_populations(raw, self.storage)

@@ -1,0 +1,126 @@
+"""The C ABI library loads and exports what include/tlbm.h declares (no GPU calls)."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tlbm.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(tlbm_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_surface():
+    names = _declared()
+    for must in ("tlbm_alloc_managed", "tlbm_kernel_lookup", "tlbm_init", "tlbm_step",
+                 "tlbm_coarsen", "tlbm_last_error", "tlbm_stream_sync"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from tlbm import _capi
+    lib = _capi.lib()
+    for name in _declared():
+        assert hasattr(lib, name), name
+    assert set(_declared()) == set(_capi.SIGNATURES), "ctypes table and header disagree"
+
+
+def test_struct_layout_matches_the_c_compiler(tmp_path):
+    """ctypes structs against offsetof/sizeof from gcc on include/tlbm.h."""
+    from tlbm import _capi
+    src = tmp_path / "layout.c"
+    src.write_text("""
+#include <stdio.h>
+#include <stddef.h>
+#include "tlbm.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(tlbm_params), offsetof(tlbm_params, qstride),
+         offsetof(tlbm_params, obs), offsetof(tlbm_params, lid), offsetof(tlbm_params, omega_rot),
+         sizeof(tlbm_spec), sizeof(tlbm_fields));
+  return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    P = _capi.Params
+    expect = [ctypes.sizeof(P), P.qstride.offset, P.obs.offset, P.lid.offset, P.omega_rot.offset,
+              ctypes.sizeof(_capi.Spec), ctypes.sizeof(_capi.Fields)]
+    assert got == expect
+
+
+def test_kernel_lookup_covers_every_combination():
+    from tlbm import _capi
+    lib = _capi.lib()
+    seen = set()
+    for st in range(3):
+        for c in range(3):
+            for o in range(3):
+                for s in range(3):
+                    for d in range(2):
+                        h = ctypes.c_int(-1)
+                        spec = _capi.Spec(c, o, d, s, st)
+                        assert lib.tlbm_kernel_lookup(ctypes.byref(spec), ctypes.byref(h)) == 0
+                        seen.add(h.value)
+    assert len(seen) == 162
+
+
+@pytest.mark.parametrize("bad", [(3, 0, 0, 0, 0), (0, -1, 0, 0, 0), (0, 0, 2, 0, 0),
+                                 (0, 0, 0, 3, 0), (0, 0, 0, 0, 3)])
+def test_kernel_lookup_rejects_unknown_specs(bad):
+    from tlbm import _capi
+    lib = _capi.lib()
+    h = ctypes.c_int(-1)
+    assert lib.tlbm_kernel_lookup(ctypes.byref(_capi.Spec(*bad)), ctypes.byref(h)) != 0
+    assert _capi.last_error()
+
+
+def test_step_argument_errors_are_reported_before_any_launch():
+    from tlbm import _capi
+    lib = _capi.lib()
+    h = ctypes.c_int(-1)
+    lib.tlbm_kernel_lookup(ctypes.byref(_capi.Spec(0, 0, 0, 0, 0)), ctypes.byref(h))
+    p = _capi.Params()
+    p.nx, p.ny, p.nz, p.qstride = 4, 4, 4, 64
+    fake_a, fake_b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    assert lib.tlbm_step(h, fake_a, fake_a, ctypes.byref(p), None, None, None) != 0
+    assert "distinct" in _capi.last_error()
+    p.nx = 0
+    assert lib.tlbm_step(h, fake_a, fake_b, ctypes.byref(p), None, None, None) != 0
+    assert "positive" in _capi.last_error()
+    p.nx, p.qstride = 4, 63
+    assert lib.tlbm_step(h, fake_a, fake_b, ctypes.byref(p), None, None, None) != 0
+    assert "qstride" in _capi.last_error()
+    p.qstride = 64
+    assert lib.tlbm_step(12345, fake_a, fake_b, ctypes.byref(p), None, None, None) != 0
+    rep = ctypes.c_int(-1)
+    lib.tlbm_kernel_lookup(ctypes.byref(_capi.Spec(0, 0, 1, 0, 0)), ctypes.byref(rep))
+    assert lib.tlbm_step(rep, fake_a, fake_b, ctypes.byref(p), None, None, None) != 0
+    assert "report" in _capi.last_error()
+    assert lib.tlbm_set_block_x(h, 96) != 0
+
+
+def test_shifted_population_codecs():
+    """encode/decode of f - w_q: f16 within half an ulp of the deviation, bf16 equal to an
+    independent round-to-nearest-even (torch's bfloat16 conversion)."""
+    import torch
+    from tlbm.lbm import W32, decode_populations, encode_populations
+    rng = np.random.default_rng(0)
+    f = (W32[:, None] + 0.02 * rng.standard_normal((19, 1000))).astype(np.float32)
+    dev = (f - W32[:, None]).astype(np.float32)
+    back = decode_populations(encode_populations(f, "f16"), "f16")
+    assert np.all(np.abs(back - f) <= np.abs(dev) * 2.0 ** -11 + 1e-7)
+    raw = encode_populations(f, "bf16")
+    ref = torch.from_numpy(dev).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(raw, ref)
+    assert np.array_equal(encode_populations(f, "f32"), f)
+    assert not encode_populations(np.tile(W32[:, None], (1, 5)), "bf16").any()

@@ -1,0 +1,187 @@
+"""Parity of the HIP step (through the C ABI) with the CPU oracle and the reference's known answer.
+
+Tolerances:
+  * FP32 storage, one step: max |gpu - oracle| <= 1e-6 * max |oracle| (FP32 arithmetic against
+    the FP64 restatement; the reference's own bar is 1e-5 absolute, trt_ref.py:60);
+  * FP16/BF16 storage: the stored deviation may differ from the oracle's exactly rounded
+    deviation by one unit in the last place (the FP32 pre-rounding value can straddle a
+    rounding boundary), i.e. |gpu - oracle| <= |oracle - w| * 2^-10 (f16) / 2^-7 (bf16) + 1e-7.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import lbm_oracle as O
+from cases import Case, WRAP_GEOMETRIES, random_fields, random_populations
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _rel(got, ref):
+    return float(np.abs(got - ref).max() / np.abs(ref).max())
+
+
+def test_trt_known_answer():
+    """trt_ref.py: one TRT(0.002) step on 8x6x4, periodic, no obstacle, no sponge."""
+    import tlbm
+    d = np.load(os.path.join(ROOT, "tests", "golden", "trt_ref_8x6x4.npz"))
+    nx, ny, nz = (int(v) for v in d["dims"])
+    lat = tlbm.D3Q19(nx, ny, nz)
+    fields = {k: tlbm.ManagedArray(lat.n) for k in tlbm.FIELDS}
+    a_in = tlbm.array(d["f_in"])
+    a_out = tlbm.empty((19 * lat.n,))
+    tlbm.lbm_step(lat, tlbm.TRT(float(d["nu"])), tlbm.Cylinder(-100.0, -100.0, 0.0),
+                  tlbm.Flow(0.0, 1e9, 0.0), tlbm.Quiet(fields), a_in, a_out)
+    got = a_out.numpy()
+    assert float(np.abs(got - d["f_out"]).max()) < float(d["tol"])
+    assert _rel(got, d["f_out"]) < 1e-6
+
+
+def _run_step(case, f64, report, storage="f32", seed=3):
+    """One step on the GPU and in the oracle from the same populations (and old fields)."""
+    import tlbm
+    lat, phys, geo, flow = case.device()
+    raw = tlbm.encode_populations(f64.astype(np.float32), storage)
+    f_in_dev = tlbm.array(raw, storage)
+    f_out_dev = tlbm.empty((19 * lat.n,), storage)
+    old = random_fields(lat.n, seed)
+    fields = {k: tlbm.ManagedArray(lat.n) for k in tlbm.FIELDS}
+    for k in tlbm.FIELDS:
+        fields[k].numpy[:] = old[k]
+    sums = tlbm.HostBuffer(14)
+    diag = tlbm.Report(fields) if report else tlbm.Quiet(fields)
+    tlbm.lbm_step(lat, phys, geo, flow, diag, f_in_dev, f_out_dev, sums=sums)
+    tlbm.sync()
+    got = tlbm.decode_populations(f_out_dev.numpy().reshape(19, lat.n), storage)
+    o_phys, o_geo, o_flow = case.oracle()
+    o_in = tlbm.decode_populations(raw, storage).astype(np.float64)
+    o_fields = {k: v.astype(np.float64) for k, v in old.items()} if report else None
+    ref = O.step(case.dims, o_phys, o_geo, o_flow, o_in, o_fields)
+    gfields = {k: fields[k].numpy.copy() for k in tlbm.FIELDS}
+    return got, ref, gfields, o_fields, sums.numpy.copy()
+
+
+COMBOS = [(c, o, s) for c in ("bgk", "trt", "les") for o in ("cylinder", "sphere", "box")
+          for s in ("periodic", "slip", "lid")]
+
+
+@pytest.mark.parametrize("collision,obstacle,side", COMBOS)
+def test_quiet_step_f32(collision, obstacle, side):
+    case = Case((24, 12, 9), collision, obstacle, side)
+    f = random_populations(case.n, seed=11)
+    got, ref, *_ = _run_step(case, f, report=False)
+    assert _rel(got, ref) < 1e-6
+
+
+@pytest.mark.parametrize("collision,obstacle,side", COMBOS)
+def test_report_step_f32(collision, obstacle, side):
+    case = Case((24, 12, 9), collision, obstacle, side)
+    f = random_populations(case.n, seed=12)
+    got, ref, g, o, sums = _run_step(case, f, report=True)
+    assert _rel(got, ref) < 1e-6
+    for k in O.FIELDS:
+        scale = max(float(np.abs(o[k]).max()), 1e-12)
+        assert float(np.abs(g[k] - o[k]).max()) <= 2e-5 * scale + 1e-9, k
+    # in-kernel FP64 sums equal FP64 sums of the fields the kernel wrote
+    for i, k in enumerate(O.FIELDS):
+        exact = float(g[k].astype(np.float64).sum())
+        assert abs(sums[i] - exact) <= 1e-12 * max(1.0, float(np.abs(g[k]).astype(np.float64).sum())), k
+    assert abs(sums[13] - float(o["_rho"].sum())) < 1e-9 * case.n
+
+
+@pytest.mark.parametrize("obstacle", ["cylinder", "sphere", "box"])
+@pytest.mark.parametrize("side", ["periodic", "slip"])
+def test_obstacle_across_periodic_x(obstacle, side):
+    case = Case((20, 11, 8), "les", obstacle, side, geometry=WRAP_GEOMETRIES[obstacle])
+    f = random_populations(case.n, seed=13)
+    got, ref, *_ = _run_step(case, f, report=True)
+    assert _rel(got, ref) < 1e-6
+
+
+@pytest.mark.parametrize("dims", [(70, 5, 3), (130, 3, 2), (257, 2, 2), (1, 4, 3), (3, 1, 1)])
+def test_ragged_and_degenerate_grids(dims):
+    """x extents that are not multiples of the block, and 1-cell-thick axes (self-periodic)."""
+    case = Case(dims, "trt", "sphere", "periodic", geometry=(1.5, 1.2, 1.1, 1.3))
+    f = random_populations(case.n, seed=14)
+    got, ref, g, o, sums = _run_step(case, f, report=True)
+    assert _rel(got, ref) < 1e-6
+
+
+@pytest.mark.parametrize("storage,bits", [("f16", 10), ("bf16", 7)])
+@pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
+@pytest.mark.parametrize("side", ["periodic", "lid"])
+def test_shifted_storage(storage, bits, collision, side):
+    case = Case((24, 12, 9), collision, "cylinder", side)
+    f = random_populations(case.n, seed=15)
+    got, ref, *_ = _run_step(case, f, report=False, storage=storage)
+    w = np.array(O.W)[:, None]
+    bound = np.abs(ref - w) * 2.0 ** -bits + 1e-7
+    assert bool(np.all(np.abs(got - ref) <= bound))
+
+
+def test_solid_cells_hold_rest_equilibrium():
+    """sigma = 1 in solids (lbm.py:449): f_out = feq(1, 0) = w_q exactly, deviation 0."""
+    import tlbm
+    case = Case((24, 12, 9), "les", "sphere", "periodic")
+    f = random_populations(case.n, seed=16)
+    for storage in ("f32", "f16", "bf16"):
+        lat, phys, geo, flow = case.device()
+        raw = tlbm.encode_populations(f.astype(np.float32), storage)
+        out = tlbm.empty((19 * lat.n,), storage)
+        tlbm.lbm_step(lat, phys, geo, flow, tlbm.Quiet({k: None for k in tlbm.FIELDS}),
+                      tlbm.array(raw, storage), out)
+        x, y, z = O.coords(*case.dims)
+        solid = O.Sphere(*case.geometry).solid(x, y, z) > 0
+        got = out.numpy().reshape(19, lat.n)[:, solid]
+        if storage == "f32":
+            assert np.array_equal(got, np.repeat(np.float32(O.W)[:, None].astype(np.float32),
+                                                 solid.sum(), axis=1))
+        else:
+            assert not got.any()
+
+
+@pytest.mark.parametrize("obstacle", ["cylinder", "sphere", "box"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_init(obstacle, storage):
+    """lbm.py:383-394 with the seeded velocity noise."""
+    import tlbm
+    case = Case((40, 14, 10), obstacle=obstacle)
+    lat, phys, geo, flow = case.device()
+    f = tlbm.empty((19 * lat.n,), storage)
+    tlbm.lbm_init(lat, geo, flow, f, 0.05, noise_amp=1e-3, noise_seed=42)
+    got = tlbm.decode_populations(f.numpy().reshape(19, lat.n), storage)
+    _, o_geo, o_flow = case.oracle()
+    ref = O.init(case.dims, o_geo, o_flow, 0.05, 1e-3, 42)
+    if storage == "f32":
+        assert _rel(got, ref) < 1e-6
+    else:
+        bits = 10 if storage == "f16" else 7
+        w = np.array(O.W)[:, None]
+        assert bool(np.all(np.abs(got - ref) <= np.abs(ref - w) * 2.0 ** -bits + 1e-7))
+
+
+def test_coarsen():
+    import tlbm
+    lat = tlbm.D3Q19(13, 9, 7)
+    src = np.random.default_rng(0).standard_normal(lat.n).astype(np.float32)
+    s = 3
+    out_dims = tuple((d + s - 1) // s for d in (13, 9, 7))
+    dst = tlbm.zeros((int(np.prod(out_dims)),))
+    tlbm.coarsen(lat, s, tlbm.array(src), dst, *out_dims)
+    assert np.array_equal(dst.numpy(), O.coarsen((13, 9, 7), s, src, out_dims))
+
+
+def test_step_rejects_bad_arguments():
+    import tlbm
+    case = Case((24, 12, 9))
+    lat, phys, geo, flow = case.device()
+    a = tlbm.empty((19 * lat.n,))
+    with pytest.raises(RuntimeError, match="distinct"):
+        tlbm.lbm_step(lat, phys, geo, flow, tlbm.Quiet({k: None for k in tlbm.FIELDS}), a, a)
+    with pytest.raises(ValueError):
+        tlbm.lbm_step(lat, phys, geo, flow, tlbm.Quiet({k: None for k in tlbm.FIELDS}), a,
+                      tlbm.empty((19 * lat.n - 19,)))
