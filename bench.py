@@ -63,7 +63,7 @@ class Events:
         return ms.value
 
 
-def time_gpu(storage, steps, warmup):
+def time_gpu(storage, steps, warmup, block_x=0, vector=-1):
     """Times `steps` steps (report every 100th) and the quiet-kernel mean from HIP events."""
     import tlbm
     from tlbm._capi import check, lib
@@ -71,6 +71,7 @@ def time_gpu(storage, steps, warmup):
     stream = ctypes.c_void_p()
     check(L.tlbm_stream_create(ctypes.byref(stream)), "stream create")
     s = make_solver(storage, stream.value)
+    s.set_launch(block_x, vector)
     for _ in range(warmup):
         s.step(report=(s.step_count % REPORT_EVERY == REPORT_EVERY - 1))
     ev = Events(L)
@@ -156,6 +157,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--storage", default="f32,f16,bf16")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=0, help="threads per block (0 = default)")
+    ap.add_argument("--vector", choices=("auto", "off"), default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is "
@@ -164,7 +167,8 @@ def main():
     storages = [s for s in args.storage.split(",") if s]
     results = {}
     for st in storages:
-        results[st] = time_gpu(st, args.steps, args.warmup)
+        results[st] = time_gpu(st, args.steps, args.warmup, args.block,
+                               -1 if args.vector == "auto" else 0)
         print(f"[bench] {st}: {results[st]['mlups']:.0f} MLUPS, quiet kernel "
               f"{results[st]['quiet_kernel_ms']:.3f} ms = {results[st]['achieved_gbs']:.0f} GB/s",
               file=sys.stderr, flush=True)

@@ -112,8 +112,11 @@ int tlbm_event_elapsed_ms(void* start, void* stop, float* ms);
 /* Resolve a specialisation (Tack's type-keyed kernel cache, README.md:55-56).  The handle is
  * stable for the life of the process. */
 int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle);
-/* Threads per block along x for step launches (64, 128 or 256); 0 restores the default. */
-int tlbm_set_block_x(int handle, int block_x);
+/* Launch shape of a specialisation: threads per block along x (64, 128 or 256; 0 = the
+ * default, the smallest that covers a row) and whether the vector kernel (4 FP32 or 8 FP16/BF16
+ * cells per thread, used when nx and qstride are multiples of that width and the buffers are
+ * 16-byte aligned) may be used (-1 = yes, 0 = one cell per thread). */
+int tlbm_set_launch(int handle, int block_x, int vector);
 
 /* lbm_init (lbm.py:383-394): f = feq(1, u0 fluid, seed_amp u0 fluid sin(2 pi x/nx)
  * cos(2 pi z/nz), 0), plus, when noise_amp != 0, a seeded uniform U(-noise_amp, noise_amp)

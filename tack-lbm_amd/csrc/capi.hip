@@ -45,7 +45,10 @@ int hip_fail(hipError_t e, const char* what) {
 constexpr int kHandles = 3 * 3 * 3 * 3 * 2;  // storage, collision, obstacle, side, diag
 
 tlbm::StepTable g_tables[3][3];
+// launch overrides per handle: threads per block (0 = default) and vector kernel use
+// (-1 = when the shape allows, 0 = never)
 int g_block_x[kHandles];
+int g_vector[kHandles];
 std::once_flag g_once;
 
 void build_tables() {
@@ -59,6 +62,7 @@ void build_tables() {
   tlbm_register_2_1(g_tables[2][1]);
   tlbm_register_2_2(g_tables[2][2]);
   for (int& b : g_block_x) b = 0;
+  for (int& v : g_vector) v = -1;
 }
 
 struct Decoded {
@@ -109,23 +113,21 @@ int workspace(size_t bytes, double** out) {
   return TLBM_OK;
 }
 
-// Sums partials[k][0..nblocks) in a fixed order: deterministic FP64 totals.
+// Sums partials[k][0..nblocks) in a fixed order (block k sums row k): deterministic FP64 totals.
 __global__ void __launch_bounds__(1024) reduce_partials(const double* __restrict__ partials,
                                                         uint32_t nblocks, double* __restrict__ out) {
   __shared__ double red[1024];
-  for (int k = 0; k < TLBM_NSUMS; ++k) {
-    double s = 0.0;
-    for (uint32_t i = threadIdx.x; i < nblocks; i += blockDim.x)
-      s += partials[static_cast<size_t>(k) * nblocks + i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[k] = red[0];
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < nblocks; i += blockDim.x)
+    s += partials[static_cast<size_t>(k) * nblocks + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[k] = red[0];
 }
 
 __global__ void coarsen_kernel(const float* __restrict__ src, float* __restrict__ dst, int nx,
@@ -180,6 +182,16 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
     a.reach[2] = std::fabs(p.obs[5]) + 2.0f;
   }
   a.r2 = r * r;
+  // periodic faces the obstacle reaches: cells on those faces test their wrapped sources
+  const float ctr[3] = {p.obs[0], p.obs[1], obstacle == TLBM_CYLINDER ? 0.0f : p.obs[2]};
+  const int ext[3] = {p.nx, p.ny, p.nz};
+  a.wrap = 0;
+  for (int d = 0; d < 3; ++d) {
+    if (d == 2 && obstacle == TLBM_CYLINDER) continue;  // infinite along z: z wraps change nothing
+    const float lo = std::fabs(0.0f - ctr[d]), hi = std::fabs(static_cast<float>(ext[d] - 1) - ctr[d]);
+    if (lo < a.reach[d] || hi < a.reach[d] || !std::isfinite(lo) || !std::isfinite(hi))
+      a.wrap |= 1 << d;
+  }
   a.u0 = p.u0;
   a.s0 = p.sponge_start;
   a.s_len = static_cast<float>(p.nx) - p.sponge_start;
@@ -189,9 +201,11 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
   return a;
 }
 
-int default_block_x(int nx) {
-  if (nx <= 64) return 64;
-  if (nx <= 128) return 128;
+// Threads per block along x for `threads_needed` threads per row: the smallest of 64/128/256
+// that covers the row, else 256.
+int default_block_x(int threads_needed) {
+  if (threads_needed <= 64) return 64;
+  if (threads_needed <= 128) return 128;
   return 256;
 }
 
@@ -367,13 +381,16 @@ int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle) {
   return TLBM_OK;
 }
 
-int tlbm_set_block_x(int handle, int block_x) {
+int tlbm_set_launch(int handle, int block_x, int vector) {
   std::call_once(g_once, build_tables);
   Decoded d;
   if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
   if (block_x != 0 && block_x != 64 && block_x != 128 && block_x != 256)
-    return fail(TLBM_ERR_INVALID, "block_x must be 64, 128 or 256 (got %d)", block_x);
+    return fail(TLBM_ERR_INVALID, "block_x must be 0, 64, 128 or 256 (got %d)", block_x);
+  if (vector != -1 && vector != 0)
+    return fail(TLBM_ERR_INVALID, "vector must be -1 (auto) or 0 (off), got %d", vector);
   g_block_x[handle] = block_x;
+  g_vector[handle] = vector;
   return TLBM_OK;
 }
 
@@ -405,9 +422,17 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
   tlbm::StepArgs a = make_args(*params, d.obstacle);
   a.fin = f_in;
   a.fout = f_out;
-  const int bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(params->nx);
+  // vector kernel: V cells per thread with aligned V-wide loads; needs nx and the plane stride
+  // to be multiples of V and 16-byte aligned buffers
+  const int vw = tlbm::vector_width(d.storage);
+  const bool vec = g_vector[handle] != 0 && params->nx % vw == 0 && params->qstride % vw == 0 &&
+                   reinterpret_cast<uintptr_t>(f_in) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(f_out) % 16 == 0;
+  const int cells = vec ? vw : 1;
+  const int threads_per_row = (params->nx + cells - 1) / cells;
+  const int bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(threads_per_row);
   const dim3 block(bx);
-  const dim3 grid((params->nx + bx - 1) / bx, params->ny, params->nz);
+  const dim3 grid((threads_per_row + bx - 1) / bx, params->ny, params->nz);
   const uint32_t nblocks = grid.x * grid.y * grid.z;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (d.diag == TLBM_REPORT) {
@@ -423,10 +448,12 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
       a.nblocks = nblocks;
     }
   }
-  hipError_t e = g_tables[d.storage][d.collision][d.obstacle][d.side][d.diag](a, grid, block, st);
+  hipError_t e =
+      g_tables[d.storage][d.collision][d.obstacle][d.side][d.diag][vec ? 1 : 0](a, grid, block, st);
   if (e != hipSuccess) return hip_fail(e, "lbm_step launch");
   if (d.diag == TLBM_REPORT && sums) {
-    hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(1024), 0, st, a.partials, nblocks, sums);
+    hipLaunchKernelGGL(reduce_partials, dim3(TLBM_NSUMS), dim3(1024), 0, st, a.partials, nblocks,
+                       sums);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "report reduction launch");
   }

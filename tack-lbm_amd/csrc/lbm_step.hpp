@@ -1,22 +1,34 @@
 // Device code of the fused D3Q19 step for gfx950.
 //
-// One thread per lattice cell, blocks laid along x so that each population plane is read and
-// written in coalesced wave-wide runs; y and z come from blockIdx, so every y/z boundary
-// decision (periodic wrap, slip mirror, lid) is uniform across the wave.  The kernel is a
-// template over collision x obstacle x side x report x storage: each combination the
-// reference's Tack compiler would specialise (lbm.py:1-7) is its own instantiation, with no
-// runtime branch on the configuration.
+// Mapping.  Each thread owns V consecutive cells of one x-row; blocks are laid along x and y, z
+// come from blockIdx, so every y/z decision (periodic wrap, slip mirror, moving lid) is uniform
+// across the wave and every row/plane offset is a scalar.  Populations are loaded and stored as
+// aligned V-wide vectors (16 bytes for FP32 V=4 and for FP16/BF16 V=8): each population plane
+// is streamed in full 64-lane x 16-byte requests.  The +-1 x offsets of the pull scheme are
+// resolved in registers: a cell's x-1 (x+1) source is the previous (next) lane's last (first)
+// element, fetched with a lane shuffle; only the first (last) lane of a wave, or a lane at a
+// periodic wrap, issues one extra scalar load.  V = 1 is the plain one-cell-per-thread form used
+// for grids whose nx is not a multiple of the vector width.
 //
-// Arithmetic follows lbm.py:397-513 with these re-arrangements, all exact or within FP32
-// rounding of the reference's expression:
-//   * walls: the solid test of a neighbour is skipped for cells whose 3x3x3 neighbourhood
-//     cannot reach the obstacle (an exact bound, see Obstacle::near); the bounced population
-//     (lbm.py:427) is loaded only on wall links, and the moving-wall term only there;
+// Specialisation.  The kernel is a template over collision x obstacle x side x report x storage
+// (and V): each combination the reference's Tack compiler would specialise (lbm.py:1-7) is its
+// own instantiation, with no runtime branch on the configuration.
+//
+// Arithmetic follows lbm.py:397-513, rearranged only in ways that are exact or within FP32
+// rounding of the reference's expressions:
+//   * walls: neighbour solid tests run only for cells whose 3x3x3 neighbourhood can reach the
+//     obstacle (an exact bound, Obstacle::near, plus the periodic wraps when the obstacle comes
+//     within reach of a domain face); the bounced population (lbm.py:427) is loaded only on wall
+//     links and the moving-wall term is formed only there;
 //   * moments: summed in the reference's q order, skipping the c = 0 terms (adding +0 is exact);
-//   * equilibria: computed once per pair (q, opp q), feq = A +- B with A = w rho (1 - 1.5 u.u
-//     + cu^2/2), B = w rho cu;
-//   * collision: post - feq written directly (BGK: (1-omega) neq; TRT: per pair
-//     (1-omega+) n+ +- (1-omega-) n-), then the sponge blend of lbm.py:446-456.
+//   * equilibria: once per pair (q, opp q): feq = A +- B, A = w rho (1 - 1.5 u.u + 4.5 (c.u)^2),
+//     B = 3 w rho c.u;
+//   * collision: post - feq written directly (BGK/LES: (1 - omega) neq; TRT: per pair
+//     (1 - omega+) n+ +- (1 - omega-) n-), then the sponge blend of lbm.py:446-456; where
+//     sigma = 0 the blend target feq(rho_s, u_s) equals feq(rho, u) bit for bit and is reused;
+//   * FP16/BF16 storage keeps deviations d_q = f_q - w_q and all arithmetic runs on them
+//     (rho = 1 + sum d, feq - w formed directly); bounce-back and the lid term are the same
+//     expressions in deviations because w_opp(q) = w_q.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -62,41 +74,49 @@ __host__ __device__ __forceinline__ void static_for(F&& f) {
 }
 
 // ---- population storage -------------------------------------------------------------------
+// value(): stored element -> the FP32 working value (f for FP32, f - w for shifted storage);
+// encode(): the inverse, round to nearest even.
 struct StoreF32 {
   using T = float;
-  static constexpr int kind = TLBM_F32;
-  __device__ static __forceinline__ float load(const T* p, uint32_t i, float) { return p[i]; }
-  __device__ static __forceinline__ void store(T* p, uint32_t i, float v, float) { p[i] = v; }
-  __device__ static __forceinline__ T encode(float v, float) { return v; }
+  static constexpr bool kShift = false;
+  static constexpr int kVec = 4;  // 16-byte vectors
+  __device__ static __forceinline__ float value(T r) { return r; }
+  __device__ static __forceinline__ T encode(float v) { return v; }
+  __device__ static __forceinline__ uint32_t bits(T r) { return __float_as_uint(r); }
+  __device__ static __forceinline__ T from_bits(uint32_t b) { return __uint_as_float(b); }
 };
 
-// Deviation from the rest equilibrium w_q, binary16, round to nearest even.
 struct StoreF16 {
   using T = _Float16;
-  static constexpr int kind = TLBM_F16;
-  __device__ static __forceinline__ float load(const T* p, uint32_t i, float w) {
-    return w + static_cast<float>(p[i]);
-  }
-  __device__ static __forceinline__ T encode(float v, float w) { return static_cast<T>(v - w); }
-  __device__ static __forceinline__ void store(T* p, uint32_t i, float v, float w) {
-    p[i] = encode(v, w);
+  static constexpr bool kShift = true;
+  static constexpr int kVec = 8;
+  __device__ static __forceinline__ float value(T r) { return static_cast<float>(r); }
+  __device__ static __forceinline__ T encode(float v) { return static_cast<T>(v); }
+  __device__ static __forceinline__ uint32_t bits(T r) { return __builtin_bit_cast(uint16_t, r); }
+  __device__ static __forceinline__ T from_bits(uint32_t b) {
+    return __builtin_bit_cast(T, static_cast<uint16_t>(b));
   }
 };
 
-// Deviation from w_q, bfloat16 (upper half of the binary32 pattern), round to nearest even.
+// bfloat16 = upper half of the binary32 pattern.
 struct StoreBF16 {
   using T = uint16_t;
-  static constexpr int kind = TLBM_BF16;
-  __device__ static __forceinline__ float load(const T* p, uint32_t i, float w) {
-    return w + __uint_as_float(static_cast<uint32_t>(p[i]) << 16);
+  static constexpr bool kShift = true;
+  static constexpr int kVec = 8;
+  __device__ static __forceinline__ float value(T r) {
+    return __uint_as_float(static_cast<uint32_t>(r) << 16);
   }
-  __device__ static __forceinline__ T encode(float v, float w) {
-    const uint32_t u = __float_as_uint(v - w);
+  __device__ static __forceinline__ T encode(float v) {
+    const uint32_t u = __float_as_uint(v);
     return static_cast<T>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
   }
-  __device__ static __forceinline__ void store(T* p, uint32_t i, float v, float w) {
-    p[i] = encode(v, w);
-  }
+  __device__ static __forceinline__ uint32_t bits(T r) { return r; }
+  __device__ static __forceinline__ T from_bits(uint32_t b) { return static_cast<T>(b); }
+};
+
+template <class T, int V>
+struct alignas(sizeof(T) * V) Pack {
+  T v[V];
 };
 
 // ---- kernel arguments (host fills them from tlbm_params) ----------------------------------
@@ -104,7 +124,8 @@ struct StepArgs {
   const void* fin;
   void* fout;
   uint64_t ns;               // plane stride (elements)
-  int32_t nx, ny, nz, pad;
+  int32_t nx, ny, nz;
+  int32_t wrap;              // bit 0/1/2: the obstacle reaches across the x/y/z periodic faces
   float omega;               // BGK
   float tau0, smag, magic;   // smag = 25.456 cs^2 (lbm.py:93)
   float o[6];                // obstacle parameters
@@ -121,7 +142,7 @@ struct StepArgs {
 // ---- obstacles: lbm.py:144-227 ---------------------------------------------------------------
 // solid() uses one rounding per subtract, multiply and add (no contraction), as a float32
 // evaluation of the reference's expression.  near() is a conservative bound: when it is false
-// for a cell that is not on a periodic wrap, none of its 19 pull sources is solid.
+// for a cell, none of its 19 pull sources that do not wrap around a periodic face is solid.
 struct Cylinder {
   static constexpr bool kMoving = true;
   __device__ static __forceinline__ float solid(const StepArgs& a, int x, int y, int) {
@@ -191,37 +212,42 @@ using StorageT = std::conditional_t<STORAGE == TLBM_F32, StoreF32,
                                     std::conditional_t<STORAGE == TLBM_F16, StoreF16, StoreBF16>>;
 
 // ---- equilibria ------------------------------------------------------------------------------
-// feq_q = w rho (1 + cu + cu^2/2 - 1.5 u.u), cu = 3 c.u  (lbm.py:52-55), for all q at once.
-__device__ __forceinline__ void equilibria(float rho, float ux, float uy, float uz, float (&e)[kQ]) {
+// feq_q = w rho (1 + cu + cu^2/2 - 1.5 u.u), cu = 3 c.u  (lbm.py:52-55), for all q at once;
+// SHIFT: feq_q - w_q instead.
+template <bool SHIFT>
+__device__ __forceinline__ void equilibria(float rho, float ux, float uy, float uz,
+                                           float (&e)[kQ]) {
   const float base = 1.0f - 1.5f * (ux * ux + uy * uy + uz * uz);
-  e[0] = kW[0] * rho * base;
+  if constexpr (SHIFT) e[0] = kW[0] * (rho * base - 1.0f);
+  else e[0] = kW[0] * rho * base;
+  const float wr1 = kW[1] * rho, wr2 = kW[7] * rho;
   static_for<0, 9>([&](auto pc) {
     constexpr int q = 2 * decltype(pc)::value + 1;
     constexpr int o = q + 1;
-    float cu = 0.0f;
-    if constexpr (kCX[q] == 1) cu += ux;
-    if constexpr (kCX[q] == -1) cu -= ux;
-    if constexpr (kCY[q] == 1) cu += uy;
-    if constexpr (kCY[q] == -1) cu -= uy;
-    if constexpr (kCZ[q] == 1) cu += uz;
-    if constexpr (kCZ[q] == -1) cu -= uz;
-    cu *= 3.0f;
-    const float wr = kW[q] * rho;
-    const float A = wr * (base + 0.5f * cu * cu);
-    const float B = wr * cu;
+    float s = 0.0f;
+    if constexpr (kCX[q] == 1) s += ux;
+    if constexpr (kCX[q] == -1) s -= ux;
+    if constexpr (kCY[q] == 1) s += uy;
+    if constexpr (kCY[q] == -1) s -= uy;
+    if constexpr (kCZ[q] == 1) s += uz;
+    if constexpr (kCZ[q] == -1) s -= uz;
+    const float wr = (q < 7) ? wr1 : wr2;
+    float A = wr * (base + 4.5f * s * s);
+    if constexpr (SHIFT) A -= kW[q];
+    const float B = (3.0f * wr) * s;
     e[q] = A + B;
     e[o] = A - B;
   });
 }
 
-// tau with the Smagorinsky eddy viscosity, lbm.py:72-93.  s[p] = neq_q + neq_opp(q) of pair p
-// (q = 2p + 1); the stress components only need these symmetric parts.
-__device__ __forceinline__ float smagorinsky_tau(const float (&neq)[kQ], float rho, float tau0,
+// tau with the Smagorinsky eddy viscosity, lbm.py:72-93, from d = f - feq.  The stress only
+// involves the pair sums d_q + d_opp(q).
+__device__ __forceinline__ float smagorinsky_tau(const float (&d)[kQ], float rho, float tau0,
                                                  float smag) {
   float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pxz = 0.0f, pyz = 0.0f;
   static_for<0, 9>([&](auto pc) {
     constexpr int q = 2 * decltype(pc)::value + 1;
-    const float s = neq[q] + neq[q + 1];
+    const float s = d[q] + d[q + 1];
     constexpr int ex = kCX[q], ey = kCY[q], ez = kCZ[q];
     if constexpr (ex != 0) pxx += s;
     if constexpr (ey != 0) pyy += s;
@@ -239,10 +265,9 @@ __device__ __forceinline__ float smagorinsky_tau(const float (&neq)[kQ], float r
 }
 
 // ---- FP64 block reduction for the report variant -------------------------------------------
-template <int BLOCK>
 __device__ __forceinline__ void block_sums(double (&v)[TLBM_NSUMS], double* partials,
                                            uint32_t nblocks, uint32_t bid) {
-  __shared__ double red[BLOCK / 64][TLBM_NSUMS];
+  __shared__ double red[4][TLBM_NSUMS];  // up to 256 threads = 4 waves
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
 #pragma unroll
@@ -261,146 +286,38 @@ __device__ __forceinline__ void block_sums(double (&v)[TLBM_NSUMS], double* part
   }
 }
 
-// ---- the fused step --------------------------------------------------------------------------
-template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE>
-__global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
-  using Obs = ObstacleT<OBS>;
-  using St = StorageT<STORAGE>;
-  using T = typename St::T;
-  const T* __restrict__ fin = static_cast<const T*>(a.fin);
-  T* __restrict__ fout = static_cast<T*>(a.fout);
-
-  const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int tx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y;
-  const int z = blockIdx.z;
-  const bool active = tx < nx;
-  if constexpr (!REPORT) {
-    if (!active) return;
-  }
-  const int x = active ? tx : nx - 1;
-  const uint32_t snx = static_cast<uint32_t>(nx);
-  const uint32_t plane = snx * static_cast<uint32_t>(ny);
-  const uint32_t idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
-
-  // pull sources along each axis (x, z periodic: lbm.py:412, 415; y by side: lbm.py:342-369)
-  const int xsrc[3] = {x == nx - 1 ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};  // cx = -1, 0, +1
-  const int zsrc[3] = {z == nz - 1 ? 0 : z + 1, z, z == 0 ? nz - 1 : z - 1};
-  const uint32_t zrow[3] = {static_cast<uint32_t>(zsrc[0]) * plane,
-                            static_cast<uint32_t>(zsrc[1]) * plane,
-                            static_cast<uint32_t>(zsrc[2]) * plane};
-  int ysrc[3];
-  bool mir_lo = false, mir_hi = false, lid_hi = false;
-  if constexpr (SIDE == TLBM_PERIODIC_Y) {
-    ysrc[0] = y == ny - 1 ? 0 : y + 1;
-    ysrc[2] = y == 0 ? ny - 1 : y - 1;
-  } else {
-    // specular reflection off the floor (and the ceiling for SlipY): the source stays in this
-    // row and the population is the y-mirrored one.  The lid replaces ceiling links instead.
-    ysrc[0] = y == ny - 1 ? y : y + 1;
-    ysrc[2] = y == 0 ? y : y - 1;
-    mir_lo = (y == 0);
-    if constexpr (SIDE == TLBM_SLIP_Y) mir_hi = (y == ny - 1);
-    else lid_hi = (y == ny - 1);
-  }
-  ysrc[1] = y;
-
-  // ---- pull streaming (lbm.py:410-426) ----
-  float f[kQ];
-  static_for<0, kQ>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-    if constexpr (SIDE == TLBM_LID_Y && cy == -1) {
-      if (lid_hi) {
-        f[q] = 0.0f;  // replaced by the lid bounce-back below
-        return;
-      }
-    }
-    const uint32_t src = zrow[1 - cz] + static_cast<uint32_t>(ysrc[1 - cy]) * snx +
-                         static_cast<uint32_t>(xsrc[1 - cx]);
-    int qs = q;
-    if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = mir_lo ? miry(q) : q;
-    if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = mir_hi ? miry(q) : q;
-    f[q] = St::load(fin + static_cast<uint64_t>(qs) * a.ns, src, kW[q]);
-  });
-
-  // ---- moving lid: f_q = f_opp(x) + 6 w_q c_q . u_lid on ceiling links ----
-  if constexpr (SIDE == TLBM_LID_Y) {
-    if (lid_hi) {
-      static_for<0, kQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        if constexpr (kCY[q] == -1) {
-          constexpr int o = opp(q);
-          const float cdu = kCX[q] * a.lid[0] + kCY[q] * a.lid[1] + kCZ[q] * a.lid[2];
-          f[q] = St::load(fin + static_cast<uint64_t>(o) * a.ns, idx, kW[o]) +
-                 6.0f * kW[q] * cdu;
-        }
-      });
-    }
-  }
-
-  // ---- obstacle: halfway / moving-wall bounce-back and momentum exchange (lbm.py:416-432) ----
-  float here = 0.0f;
-  float mx = 0.0f, my = 0.0f, mz = 0.0f;
-  bool wrap = (x == 0) || (x == nx - 1) || (z == 0) || (z == nz - 1);
-  if constexpr (SIDE == TLBM_PERIODIC_Y) wrap = wrap || (y == 0) || (y == ny - 1);
-  if (wrap || Obs::near(a, x, y, z)) {
-    here = Obs::solid(a, x, y, z);
-    const float fluid = 1.0f - here;
-    float omega_rot = 0.0f;
-    if constexpr (Obs::kMoving) omega_rot = a.omega_rot ? *a.omega_rot : 0.0f;
-    static_for<1, kQ>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-      constexpr int o = opp(q);
-      if constexpr (SIDE == TLBM_LID_Y && cy == -1) {
-        if (lid_hi) return;
-      }
-      const float wall = Obs::solid(a, xsrc[1 - cx], ysrc[1 - cy], zsrc[1 - cz]);
-      if (wall != 0.0f) {
-        const float b = St::load(fin + static_cast<uint64_t>(o) * a.ns, idx, kW[o]);
-        float t = 0.0f;
-        if constexpr (Obs::kMoving) {
-          // wall velocity at the link midpoint (lbm.py:418-425)
-          float uwx, uwy, uwz;
-          Obs::wall_vel(a, static_cast<float>(x) - 0.5f * cx, static_cast<float>(y) - 0.5f * cy,
-                        static_cast<float>(z) - 0.5f * cz, uwx, uwy, uwz);
-          t = 6.0f * kW[q] * (cx * (omega_rot * uwx) + cy * (omega_rot * uwy) +
-                              cz * (omega_rot * uwz));
-        }
-        f[q] = b + t;
-        const float hit = fluid * (2.0f * b + t);
-        mx += hit * static_cast<float>(kCX[o]);
-        my += hit * static_cast<float>(kCY[o]);
-        mz += hit * static_cast<float>(kCZ[o]);
-      }
-    });
-  }
-  const float fluid = 1.0f - here;
-
-  // ---- moments (lbm.py:433-444) ----
+// ---- one cell's collision and sponge (lbm.py:433-456) ------------------------------------------
+// v: the 19 streamed working values in, the values to store out.
+template <int COLL, bool SHIFT>
+__device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], float x,
+                                             float here, float& rho_out, float& ux_out,
+                                             float& uy_out, float& uz_out) {
+  // moments (lbm.py:433-444); shifted storage: rho = 1 + sum d
   float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
   static_for<0, kQ>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    rho += f[q];
-    if constexpr (kCX[q] == 1) jx += f[q];
-    if constexpr (kCX[q] == -1) jx -= f[q];
-    if constexpr (kCY[q] == 1) jy += f[q];
-    if constexpr (kCY[q] == -1) jy -= f[q];
-    if constexpr (kCZ[q] == 1) jz += f[q];
-    if constexpr (kCZ[q] == -1) jz -= f[q];
+    rho += v[q];
+    if constexpr (kCX[q] == 1) jx += v[q];
+    if constexpr (kCX[q] == -1) jx -= v[q];
+    if constexpr (kCY[q] == 1) jy += v[q];
+    if constexpr (kCY[q] == -1) jy -= v[q];
+    if constexpr (kCZ[q] == 1) jz += v[q];
+    if constexpr (kCZ[q] == -1) jz -= v[q];
   });
-  const float ux = fluid * jx / rho;
-  const float uy = fluid * jy / rho;
-  const float uz = fluid * jz / rho;
+  if constexpr (SHIFT) rho += 1.0f;
+  const float fluid = 1.0f - here;
+  const float inv_rho = fluid / rho;
+  const float ux = jx * inv_rho;
+  const float uy = jy * inv_rho;
+  const float uz = jz * inv_rho;
 
-  // ---- collision: d[q] = post_q - feq_q (lbm.py:60-139) ----
+  // collision: d[q] = post_q - feq_q (lbm.py:60-139)
   float e[kQ];
-  equilibria(rho, ux, uy, uz, e);
+  equilibria<SHIFT>(rho, ux, uy, uz, e);
   float d[kQ];
   static_for<0, kQ>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    d[q] = f[q] - e[q];
+    d[q] = v[q] - e[q];
   });
   if constexpr (COLL == TLBM_BGK) {
     const float k = 1.0f - a.omega;
@@ -423,63 +340,250 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
     });
   }
 
-  // ---- sponge and store (lbm.py:446-456) ----
-  const float t = fmaxf(0.0f, (static_cast<float>(x) - a.s0) / a.s_len);
+  // sponge toward the inflow state (lbm.py:446-456)
+  const float t = fmaxf(0.0f, (x - a.s0) / a.s_len);
   const float sigma = fmaxf(here, a.s_k * t * t);
-  const float rho_s = rho + sigma * (1.0f - rho);
-  const float ux_s = ux + sigma * (a.u0 * fluid - ux);
-  const float uy_s = (1.0f - sigma) * uy;
-  const float uz_s = (1.0f - sigma) * uz;
-  float es[kQ];
-  equilibria(rho_s, ux_s, uy_s, uz_s, es);
-  const float keep = 1.0f - sigma;
+  if (sigma != 0.0f) {
+    const float rho_s = rho + sigma * (1.0f - rho);
+    const float ux_s = ux + sigma * (a.u0 * fluid - ux);
+    const float uy_s = (1.0f - sigma) * uy;
+    const float uz_s = (1.0f - sigma) * uz;
+    float es[kQ];
+    equilibria<SHIFT>(rho_s, ux_s, uy_s, uz_s, es);
+    const float keep = 1.0f - sigma;
+    static_for<0, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      v[q] = es[q] + keep * d[q];
+    });
+  } else {
+    static_for<0, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      v[q] = e[q] + d[q];
+    });
+  }
+  rho_out = rho;
+  ux_out = ux;
+  uy_out = uy;
+  uz_out = uz;
+}
+
+// ---- the fused step --------------------------------------------------------------------------
+template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, int V>
+__global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
+  using Obs = ObstacleT<OBS>;
+  using St = StorageT<STORAGE>;
+  using T = typename St::T;
+  constexpr bool kShift = St::kShift;
+  const T* __restrict__ fin = static_cast<const T*>(a.fin);
+  T* __restrict__ fout = static_cast<T*>(a.fout);
+
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int lane = threadIdx.x & 63;
+  const int x0raw = (blockIdx.x * blockDim.x + threadIdx.x) * V;
+  const int y = blockIdx.y;
+  const int z = blockIdx.z;
+  const bool active = x0raw < nx;
+  if constexpr (!REPORT && V == 1) {
+    if (!active) return;
+  }
+  // inactive lanes (past the end of the row) mirror the last cells so that every address stays
+  // in bounds and every lane takes part in the shuffles; they store nothing
+  const int x0 = active ? x0raw : nx - V;
+  const uint32_t snx = static_cast<uint32_t>(nx);
+  const uint32_t plane = snx * static_cast<uint32_t>(ny);
+  const uint32_t row0 = (static_cast<uint32_t>(z) * ny + y) * snx;
+
+  // pull source rows (x, z periodic: lbm.py:412, 415; y by side: lbm.py:342-369), indexed
+  // [1 + c]: the source of a population moving with +c is one cell back.
+  const int zsrc[3] = {z == nz - 1 ? 0 : z + 1, z, z == 0 ? nz - 1 : z - 1};
+  int ysrc[3];
+  bool mir_lo = false, mir_hi = false, lid_hi = false;
+  if constexpr (SIDE == TLBM_PERIODIC_Y) {
+    ysrc[0] = y == ny - 1 ? 0 : y + 1;
+    ysrc[2] = y == 0 ? ny - 1 : y - 1;
+  } else {
+    // specular reflection off the floor (and the ceiling for SlipY): the source stays in this
+    // row and the population is the y-mirrored one.  The lid replaces ceiling links instead.
+    ysrc[0] = y == ny - 1 ? y : y + 1;
+    ysrc[2] = y == 0 ? y : y - 1;
+    mir_lo = (y == 0);
+    if constexpr (SIDE == TLBM_SLIP_Y) mir_hi = (y == ny - 1);
+    else lid_hi = (y == ny - 1);
+  }
+  ysrc[1] = y;
+
+  // ---- pull streaming (lbm.py:410-426) ----
+  float f[V][kQ];
+  static_for<0, kQ>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+    if constexpr (SIDE == TLBM_LID_Y && cy == -1) {
+      if (lid_hi) return;  // replaced by the lid bounce-back below
+    }
+    int qs = q;
+    if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = mir_lo ? miry(q) : q;
+    if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = mir_hi ? miry(q) : q;
+    const T* __restrict__ row = fin + static_cast<uint64_t>(qs) * a.ns +
+                                static_cast<uint32_t>(zsrc[1 + cz]) * plane +
+                                static_cast<uint32_t>(ysrc[1 + cy]) * snx;
+    if constexpr (V == 1) {
+      const int xs = cx == 1    ? (x0 == 0 ? nx - 1 : x0 - 1)
+                     : cx == -1 ? (x0 == nx - 1 ? 0 : x0 + 1)
+                                : x0;
+      f[0][q] = St::value(row[xs]);
+    } else {
+      const Pack<T, V> A = *reinterpret_cast<const Pack<T, V>*>(row + x0);
+      if constexpr (cx == 0) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) f[k][q] = St::value(A.v[k]);
+      } else if constexpr (cx == 1) {
+        // x - 1 of the first cell: last element of the previous lane
+        const uint32_t b =
+            static_cast<uint32_t>(__shfl_up(static_cast<int>(St::bits(A.v[V - 1])), 1, 64));
+        T prev = St::from_bits(b);
+        if (lane == 0 || x0 == 0) prev = row[x0 == 0 ? nx - 1 : x0 - 1];
+        f[0][q] = St::value(prev);
+#pragma unroll
+        for (int k = 1; k < V; ++k) f[k][q] = St::value(A.v[k - 1]);
+      } else {
+        // x + 1 of the last cell: first element of the next lane
+        const uint32_t b =
+            static_cast<uint32_t>(__shfl_down(static_cast<int>(St::bits(A.v[0])), 1, 64));
+        T next = St::from_bits(b);
+        if (lane == 63 || x0 + V == nx) next = row[x0 + V == nx ? 0 : x0 + V];
+#pragma unroll
+        for (int k = 0; k < V - 1; ++k) f[k][q] = St::value(A.v[k + 1]);
+        f[V - 1][q] = St::value(next);
+      }
+    }
+  });
+
+  // ---- moving lid: f_q = f_opp(x) + 6 w_q c_q . u_lid on ceiling links ----
+  if constexpr (SIDE == TLBM_LID_Y) {
+    if (lid_hi) {
+      static_for<0, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (kCY[q] == -1) {
+          constexpr int o = opp(q);
+          const float t =
+              6.0f * kW[q] * (kCX[q] * a.lid[0] + kCY[q] * a.lid[1] + kCZ[q] * a.lid[2]);
+          const Pack<T, V> B = *reinterpret_cast<const Pack<T, V>*>(
+              fin + static_cast<uint64_t>(o) * a.ns + row0 + x0);
+#pragma unroll
+          for (int k = 0; k < V; ++k) f[k][q] = St::value(B.v[k]) + t;
+        }
+      });
+    }
+  }
+
+  // ---- obstacle: halfway / moving-wall bounce-back and momentum exchange (lbm.py:416-432) ----
+  float here[V], mx[V], my[V], mz[V];
+  bool wrap_yz = (a.wrap & 4) && (z == 0 || z == nz - 1);
+  if constexpr (SIDE == TLBM_PERIODIC_Y)
+    wrap_yz = wrap_yz || ((a.wrap & 2) && (y == 0 || y == ny - 1));
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    here[k] = 0.0f;
+    mx[k] = my[k] = mz[k] = 0.0f;
+    const int x = x0 + k;
+    const bool wrap = wrap_yz || ((a.wrap & 1) && (x == 0 || x == nx - 1));
+    if (wrap || Obs::near(a, x, y, z)) {
+      here[k] = Obs::solid(a, x, y, z);
+      const float fluid = 1.0f - here[k];
+      float omega_rot = 0.0f;
+      if constexpr (Obs::kMoving) omega_rot = a.omega_rot ? *a.omega_rot : 0.0f;
+      const int xsrc[3] = {x == nx - 1 ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};
+      static_for<1, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+        constexpr int o = opp(q);
+        if constexpr (SIDE == TLBM_LID_Y && cy == -1) {
+          if (lid_hi) return;
+        }
+        const float wall = Obs::solid(a, xsrc[1 + cx], ysrc[1 + cy], zsrc[1 + cz]);
+        if (wall != 0.0f) {
+          const float b = St::value(fin[static_cast<uint64_t>(o) * a.ns + row0 + x]);
+          float t = 0.0f;
+          if constexpr (Obs::kMoving) {
+            // wall velocity at the link midpoint (lbm.py:418-425)
+            float uwx, uwy, uwz;
+            Obs::wall_vel(a, static_cast<float>(x) - 0.5f * cx,
+                          static_cast<float>(y) - 0.5f * cy, static_cast<float>(z) - 0.5f * cz,
+                          uwx, uwy, uwz);
+            t = 6.0f * kW[q] *
+                (cx * (omega_rot * uwx) + cy * (omega_rot * uwy) + cz * (omega_rot * uwz));
+          }
+          f[k][q] = b + t;
+          if constexpr (REPORT) {
+            const float hit = fluid * (2.0f * (kShift ? b + kW[o] : b) + t);
+            mx[k] += hit * static_cast<float>(kCX[o]);
+            my[k] += hit * static_cast<float>(kCY[o]);
+            mz[k] += hit * static_cast<float>(kCZ[o]);
+          }
+        }
+      });
+    }
+  }
+
+  // ---- collision, sponge, store ----
+  float rho[V], ux[V], uy[V], uz[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+    collide_cell<COLL, kShift>(a, f[k], static_cast<float>(x0 + k), here[k], rho[k], ux[k],
+                               uy[k], uz[k]);
   if (active) {
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      St::store(fout + static_cast<uint64_t>(q) * a.ns, idx, es[q] + keep * d[q], kW[q]);
+      Pack<T, V> out;
+#pragma unroll
+      for (int k = 0; k < V; ++k) out.v[k] = St::encode(f[k][q]);
+      *reinterpret_cast<Pack<T, V>*>(fout + static_cast<uint64_t>(q) * a.ns + row0 + x0) = out;
     });
   }
 
   // ---- report fields and FP64 sums (lbm.py:493-513, 649-661) ----
   if constexpr (REPORT) {
     const tlbm_fields& F = a.fld;
-    const float p = (rho - 1.0f) / 3.0f;
-    const float usq = ux * ux + uy * uy + uz * uz;
-    const float ke = 0.5f * rho * usq;
-    double v[TLBM_NSUMS];
+    double acc[TLBM_NSUMS];
+#pragma unroll
+    for (int i = 0; i < TLBM_NSUMS; ++i) acc[i] = 0.0;
     if (active) {
-      const float ox = F.ux[idx], oy = F.uy[idx], oz = F.uz[idx], op = F.p[idx];
-      const float ok = 0.5f * (ox * ox + oy * oy + oz * oz);
-      const float r_rho = (3.0f * (p - op)) * (3.0f * (p - op));
-      const float r_ux = (ux - ox) * (ux - ox);
-      const float r_uy = (uy - oy) * (uy - oy);
-      const float r_uz = (uz - oz) * (uz - oz);
-      const float r_ke = (ke - ok) * (ke - ok);
-      const float vmag = sqrtf(usq);
-      F.r_rho[idx] = r_rho;
-      F.r_ux[idx] = r_ux;
-      F.r_uy[idx] = r_uy;
-      F.r_uz[idx] = r_uz;
-      F.r_ke[idx] = r_ke;
-      F.ux[idx] = ux;
-      F.uy[idx] = uy;
-      F.uz[idx] = uz;
-      F.p[idx] = p;
-      F.vmag[idx] = vmag;
-      F.fx[idx] = mx;
-      F.fy[idx] = my;
-      F.fz[idx] = mz;
-      const float vals[TLBM_NSUMS] = {ux,   uy,   uz,   p,  vmag, r_rho, r_ux,
-                                      r_uy, r_uz, r_ke, mx, my,   mz,    rho};
 #pragma unroll
-      for (int k = 0; k < TLBM_NSUMS; ++k) v[k] = static_cast<double>(vals[k]);
-    } else {
+      for (int k = 0; k < V; ++k) {
+        const uint32_t idx = row0 + x0 + k;
+        const float p = (rho[k] - 1.0f) / 3.0f;
+        const float usq = ux[k] * ux[k] + uy[k] * uy[k] + uz[k] * uz[k];
+        const float ke = 0.5f * rho[k] * usq;
+        const float ox = F.ux[idx], oy = F.uy[idx], oz = F.uz[idx], op = F.p[idx];
+        const float ok = 0.5f * (ox * ox + oy * oy + oz * oz);
+        const float r_rho = (3.0f * (p - op)) * (3.0f * (p - op));
+        const float r_ux = (ux[k] - ox) * (ux[k] - ox);
+        const float r_uy = (uy[k] - oy) * (uy[k] - oy);
+        const float r_uz = (uz[k] - oz) * (uz[k] - oz);
+        const float r_ke = (ke - ok) * (ke - ok);
+        const float vmag = sqrtf(usq);
+        F.r_rho[idx] = r_rho;
+        F.r_ux[idx] = r_ux;
+        F.r_uy[idx] = r_uy;
+        F.r_uz[idx] = r_uz;
+        F.r_ke[idx] = r_ke;
+        F.ux[idx] = ux[k];
+        F.uy[idx] = uy[k];
+        F.uz[idx] = uz[k];
+        F.p[idx] = p;
+        F.vmag[idx] = vmag;
+        F.fx[idx] = mx[k];
+        F.fy[idx] = my[k];
+        F.fz[idx] = mz[k];
+        const float vals[TLBM_NSUMS] = {ux[k], uy[k], uz[k], p,     vmag,  r_rho, r_ux,
+                                        r_uy,  r_uz,  r_ke,  mx[k], my[k], mz[k], rho[k]};
 #pragma unroll
-      for (int k = 0; k < TLBM_NSUMS; ++k) v[k] = 0.0;
+        for (int i = 0; i < TLBM_NSUMS; ++i) acc[i] += static_cast<double>(vals[i]);
+      }
     }
     if (a.partials) {
       const uint32_t bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-      block_sums<256>(v, a.partials, a.nblocks, bid);
+      block_sums(acc, a.partials, a.nblocks, bid);
     }
   }
 }
@@ -509,7 +613,8 @@ __global__ void __launch_bounds__(256) lbm_init_kernel(const StepArgs a, float s
   const float fluid = 1.0f - Obs::solid(a, x, y, z);
   const float two_pi = 6.2831853f;
   float ux = a.u0 * fluid;
-  float uy = seed_amp * a.u0 * fluid * sinf(two_pi * static_cast<float>(x) / static_cast<float>(a.nx)) *
+  float uy = seed_amp * a.u0 * fluid *
+             sinf(two_pi * static_cast<float>(x) / static_cast<float>(a.nx)) *
              cosf(two_pi * static_cast<float>(z) / static_cast<float>(a.nz));
   float uz = 0.0f;
   if (noise_amp != 0.0f) {
@@ -518,10 +623,10 @@ __global__ void __launch_bounds__(256) lbm_init_kernel(const StepArgs a, float s
     uz += fluid * (noise_amp * (2.0f * hash_uniform(noise_seed, idx, 2) - 1.0f));
   }
   float e[kQ];
-  equilibria(1.0f, ux, uy, uz, e);
+  equilibria<St::kShift>(1.0f, ux, uy, uz, e);
   static_for<0, kQ>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    St::store(f + static_cast<uint64_t>(q) * a.ns, idx, e[q], kW[q]);
+    f[static_cast<uint64_t>(q) * a.ns + idx] = St::encode(e[q]);
   });
 }
 

@@ -10,8 +10,12 @@ arguments.  ``lbm_step`` / ``lbm_init`` / ``coarsen`` keep the reference's kerne
 Differences, by design:
   * the step runs on libtlbm.so (hand-written HIP for gfx950); there is no CPU path;
   * ``Solver.monitors`` reads FP64 sums reduced inside the report kernel instead of summing the
-    13 managed fields on the host (lbm.py:654); the fields are still written, so in-situ
-    readers see the same zero-copy arrays;
+    13 managed fields on the host (lbm.py:654);
+  * the 13 report fields live in HBM by default (``field_memory="device"``): managed memory
+    on an MI355X without XNACK is host-resident, and writing the fields there each report step
+    would cost ~30x the step itself.  ``Solver.output`` stays a set of managed arrays that
+    in-situ readers use zero-copy; ``update_output()`` (called by ``monitors``) refreshes them.
+    ``field_memory="managed"`` keeps the reference's layout (the kernel writes managed memory);
   * population storage can be FP16 or BF16 deviations f_q - w_q (``storage="f16"/"bf16"``);
   * ``LidY`` adds a moving lid; obstacle centre, sponge start, Cs/magic and a seeded velocity
     noise can be overridden;
@@ -361,6 +365,15 @@ def kernel_handle(collision, obstacle, diag, side, storage):
     return h
 
 
+def set_launch(phys, geo, diag, side, storage, block_x=0, vector=-1):
+    """Launch shape of one specialisation (process-wide): threads per block along x (0 = the
+    default) and whether the vector kernel may be used (-1 = when the shape allows, 0 = never)."""
+    h = kernel_handle(_code(phys, "collision"), _code(geo, "obstacle"),
+                      _capi.REPORT if getattr(diag, "enabled", 0) == 1 else _capi.QUIET,
+                      _code(side, "side"), STORAGES[storage])
+    check(lib().tlbm_set_launch(h, int(block_x), int(vector)), "set launch")
+
+
 def _code(obj, what):
     code = getattr(type(obj), "code", None)
     if code is None:
@@ -474,7 +487,7 @@ class Solver:
                  heat=False, prandtl=0.7, lateral="periodic", *, storage="f32",
                  lid_u=(0.04, 0.0, 0.0), center=None, sponge_start=None, sponge_strength=1.0,
                  cs=None, magic=None, seed_amp=0.05, noise_amp=0.0, noise_seed=0,
-                 qstride=None, stream=None):
+                 qstride=None, stream=None, field_memory="device"):
         if heat:
             raise NotImplementedError("the D3Q7 heat variant (lbm.py:279-319) is not built")
         _select_device(device)
@@ -498,7 +511,17 @@ class Solver:
         if self.qstride < n:
             raise ValueError(f"qstride {self.qstride} < cells {n}")
         self.f = [empty((19 * self.qstride,), storage), empty((19 * self.qstride,), storage)]
-        self.fields = {name: ManagedArray(n) for name in FIELDS}
+        # Report fields.  "device": HBM, written by the report kernel at full bandwidth, with
+        # host-visible managed copies of the output fields refreshed by update_output().
+        # "managed": the reference's layout (lbm.py:567), the kernel writes managed memory
+        # directly; on devices without XNACK/HMM that memory is host-resident and slow to write.
+        if field_memory not in ("device", "managed"):
+            raise ValueError(f"field_memory must be 'device' or 'managed', got {field_memory!r}")
+        self.field_memory = field_memory
+        if field_memory == "device":
+            self.fields = {name: zeros((n,)) for name in FIELDS}
+        else:
+            self.fields = {name: ManagedArray(n) for name in FIELDS}
         self.quiet = Quiet(self.fields)
         self.report = Report(self.fields)
         self.sums = HostBuffer(_capi.TLBM_NSUMS)
@@ -507,7 +530,7 @@ class Solver:
         self.prandtl = prandtl
         self.stride = stride
         self.out_dims = tuple((d + stride - 1) // stride for d in (nx, ny, nz))
-        if stride > 1:
+        if stride > 1 or field_memory == "device":
             m = self.out_dims[0] * self.out_dims[1] * self.out_dims[2]
             self.output = {name: ManagedArray(m) for name in self.OUTPUT}
         else:
@@ -518,12 +541,22 @@ class Solver:
         self.step_count = 0
 
     def update_output(self):
-        """lbm.py:606-611."""
+        """lbm.py:606-611: refresh the host-visible output arrays (coarsened when stride > 1)."""
         if self.stride > 1:
             mx, my, mz = self.out_dims
             for name in self.output:
                 coarsen(self.lat, self.stride, self.fields[name], self.output[name], mx, my, mz,
                         self.stream)
+        elif self.field_memory == "device":
+            for name in self.output:
+                src = self.fields[name]
+                check(lib().tlbm_memcpy_async(self.output[name].ptr, src.ptr, src.nbytes,
+                                              self.stream), "output copy")
+
+    def set_launch(self, block_x=0, vector=-1):
+        """Launch shape of this solver's quiet and report kernels (see ``set_launch``)."""
+        for diag in (self.quiet, self.report):
+            set_launch(self.phys, self.geo, diag, self.flow.side, self.storage, block_x, vector)
 
     def set_physics(self, collision, reynolds):
         """lbm.py:613-616: nu = u0 D / Re; a collision change selects another kernel."""
