@@ -1,0 +1,23 @@
+"""The workload configurations of the parity and physics checks, shared by tests and fixture
+generators (each is a keyword set for ``Solver`` / ``OracleSolver``)."""
+
+# Plain method: D3Q19 BGK 96x48x48 channel, cylinder (axis z) radius 6 at x = 24, y = 24,
+# u0 = 0.08, Re 100 (D = 12), seeded U(+-1e-3) velocity noise, sponge over the last 8 planes,
+# 1000 steps.
+C1 = dict(nx=96, ny=48, nz=48, collision="bgk", obstacle="cylinder", reynolds=100.0, u0=0.08,
+          diameter=12.0, center=(24.0, 24.0, 24.0), sponge_start=88.0, noise_amp=1e-3,
+          noise_seed=0)
+C1_STEPS = 1000
+
+# TRT + Smagorinsky (Cs 0.1, magic 1/4), 256x128x128, cylinder radius 12 at the reference's
+# default placement, u0 = 0.08, Re 300, seeded noise, 20000 steps (Strouhal from the lift).
+C2 = dict(nx=256, ny=128, nz=128, collision="trt", obstacle="cylinder", reynolds=300.0,
+          u0=0.08, diameter=24.0, cs=0.1, magic=0.25, noise_amp=1e-3, noise_seed=0)
+C2_STEPS = 20000
+C2_SAMPLE_EVERY = 10
+
+# LES 512x256x256, sphere radius 16 at x = 96, moving lid u = 0.04 on a free-slip floor,
+# Re 3000 (D = 32), report every 100 steps (the headline benchmark workload).
+C3 = dict(nx=512, ny=256, nz=256, collision="les", obstacle="sphere", reynolds=3000.0, u0=0.08,
+          diameter=32.0, center=(96.0, 128.0, 128.0), lateral="lid", lid_u=(0.04, 0.0, 0.0),
+          noise_amp=1e-3, noise_seed=0)

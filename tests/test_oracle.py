@@ -356,3 +356,54 @@ def test_oracle_solver_first_report_matches_construction():
     s = O.OracleSolver(24, 12, 8, "bgk", "cylinder", 100.0, 0.08, 6.0)
     m = s.monitors(1)
     assert np.isfinite(m["drag"]) and s.step_count == 0
+
+
+# --- the C restatement (oracle/lbm_oracle.c) against the NumPy one --------------------------------
+def _c_oracle():
+    import subprocess
+    from oracle import c_oracle
+    if not c_oracle.available():
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True,
+                       capture_output=True)
+    return c_oracle
+
+
+@pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
+@pytest.mark.parametrize("obstacle", ["cylinder", "sphere", "box"])
+@pytest.mark.parametrize("side", ["periodic", "slip", "lid"])
+def test_c_oracle_matches_numpy_oracle(collision, obstacle, side):
+    C_ = _c_oracle()
+    dims = (24, 12, 9)
+    n = 24 * 12 * 9
+    f = O.W[:, None] + 0.01 * np.random.default_rng(9).standard_normal((19, n))
+    phys = {"bgk": O.BGK(0.02), "trt": O.TRT(0.01, 0.25, 0.1), "les": O.LES(0.01)}[collision]
+    geo = {"cylinder": O.Cylinder(9.3, 6.2, 3.1), "sphere": O.Sphere(9.3, 6.2, 4.4, 3.1),
+           "box": O.Box(0.4, 6.2, 4.4, 2.6, 2.1, 2.4)}[obstacle]
+    side_obj = O.LidY(0.04, 0.01, -0.02) if side == "lid" else O.SIDES[side]()
+    flow = O.Flow(0.08, 12.0, 1.0, 0.37, side_obj)
+    d1, d2 = O.new_fields(n), O.new_fields(n)
+    d1["ux"][:] = d2["ux"][:] = 0.01
+    a = O.step(dims, phys, geo, flow, f, d1)
+    b = C_.step(dims, phys, geo, flow, f, d2)
+    assert np.abs(a - b).max() <= 1e-15
+    for k in O.FIELDS:
+        assert np.abs(d1[k] - d2[k]).max() <= 1e-15, k
+    assert np.abs(d1["_rho"] - d2["_rho"]).max() <= 1e-15
+
+
+def test_c_oracle_init_and_solver_match():
+    C_ = _c_oracle()
+    kw = dict(collision="trt", obstacle="sphere", reynolds=200.0, diameter=6.0, noise_amp=1e-3,
+              noise_seed=3, lateral="slip", cs=0.1, magic=0.25)
+    a = O.OracleSolver(32, 16, 12, **kw)
+    b = C_.CSolver(32, 16, 12, **kw)
+    assert np.abs(a.f - b.f).max() <= 1e-15
+    for _ in range(3):
+        a.step()
+        b.step()
+    a.step(report=True)
+    b.step(report=True)
+    assert np.abs(a.f - b.f).max() <= 1e-14
+    ma, mb = a.monitors(4), b.monitors(4)
+    for k in ma:
+        assert abs(ma[k] - mb[k]) <= 1e-12 * max(1.0, abs(ma[k])), k
