@@ -63,7 +63,7 @@ class Events:
         return ms.value
 
 
-def time_gpu(storage, steps, warmup, block_x=0, vector=-1):
+def time_gpu(storage, steps, warmup, block_x=0):
     """Times `steps` steps (report every 100th) and the quiet-kernel mean from HIP events."""
     import tlbm
     from tlbm._capi import check, lib
@@ -71,7 +71,7 @@ def time_gpu(storage, steps, warmup, block_x=0, vector=-1):
     stream = ctypes.c_void_p()
     check(L.tlbm_stream_create(ctypes.byref(stream)), "stream create")
     s = make_solver(storage, stream.value)
-    s.set_launch(block_x, vector)
+    s.set_launch(block_x)
     for _ in range(warmup):
         s.step(report=(s.step_count % REPORT_EVERY == REPORT_EVERY - 1))
     ev = Events(L)
@@ -127,6 +127,27 @@ def time_gpu(storage, steps, warmup, block_x=0, vector=-1):
     return out
 
 
+def calibrate_copy(gib=1, reps=5):
+    """A plain copy of `gib` GiB (read + write) with the library's copy kernel: the known-size
+    reference for HBM counters and the achievable streaming bandwidth on this device."""
+    import tlbm
+    from tlbm._capi import check, lib
+    L = lib()
+    nbytes = gib << 30
+    a = tlbm.empty((nbytes // 4,))
+    b = tlbm.empty((nbytes // 4,))
+    check(L.tlbm_copy(b.ptr, a.ptr, nbytes, None), "copy")
+    ev = Events(L)
+    e0, e1 = ev.new(), ev.new()
+    check(L.tlbm_event_record(e0, None), "record")
+    for _ in range(reps):
+        check(L.tlbm_copy(b.ptr, a.ptr, nbytes, None), "copy")
+    check(L.tlbm_event_record(e1, None), "record")
+    ms = ev.elapsed(e0, e1) / reps
+    return {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms,
+            "gbs": 2 * nbytes / (ms * 1e-3) / 1e9}
+
+
 def cpu_baseline(max_seconds=20.0):
     """The NumPy float32 restatement on a 512x256x4 slab of the workload, one core."""
     import numpy as np
@@ -158,7 +179,8 @@ def main():
     ap.add_argument("--storage", default="f32,f16,bf16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0 = default)")
-    ap.add_argument("--vector", choices=("auto", "off"), default="auto")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also time a plain 1 GiB copy (counter calibration, streaming ceiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is "
@@ -167,11 +189,13 @@ def main():
     storages = [s for s in args.storage.split(",") if s]
     results = {}
     for st in storages:
-        results[st] = time_gpu(st, args.steps, args.warmup, args.block,
-                               -1 if args.vector == "auto" else 0)
+        results[st] = time_gpu(st, args.steps, args.warmup, args.block)
         print(f"[bench] {st}: {results[st]['mlups']:.0f} MLUPS, quiet kernel "
               f"{results[st]['quiet_kernel_ms']:.3f} ms = {results[st]['achieved_gbs']:.0f} GB/s",
               file=sys.stderr, flush=True)
+    copy = calibrate_copy() if args.calibrate else None
+    if copy:
+        print(f"[bench] copy kernel: {copy['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
     head = results.get("f32") or results[storages[0]]
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
@@ -201,6 +225,9 @@ def main():
                           "drag": r["monitors"]["drag"], "lift": r["monitors"]["lift"]}
                      for st, r in results.items()},
     }
+    if copy:
+        line["copy_calibration"] = {k: round(v, 4) if isinstance(v, float) else v
+                                    for k, v in copy.items()}
     print(json.dumps(line), flush=True)
 
 

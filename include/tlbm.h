@@ -112,11 +112,9 @@ int tlbm_event_elapsed_ms(void* start, void* stop, float* ms);
 /* Resolve a specialisation (Tack's type-keyed kernel cache, README.md:55-56).  The handle is
  * stable for the life of the process. */
 int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle);
-/* Launch shape of a specialisation: threads per block along x (64, 128 or 256; 0 = the
- * default, the smallest that covers a row) and whether the vector kernel (4 FP32 or 8 FP16/BF16
- * cells per thread, used when nx and qstride are multiples of that width and the buffers are
- * 16-byte aligned) may be used (-1 = yes, 0 = one cell per thread). */
-int tlbm_set_launch(int handle, int block_x, int vector);
+/* Launch shape of a specialisation: threads per block along x (64, 128 or 256; 0 restores the
+ * default: the smallest of 64/128 covering a row, else 256 for FP32 and 128 for FP16/BF16). */
+int tlbm_set_launch(int handle, int block_x);
 
 /* lbm_init (lbm.py:383-394): f = feq(1, u0 fluid, seed_amp u0 fluid sin(2 pi x/nx)
  * cos(2 pi z/nz), 0), plus, when noise_amp != 0, a seeded uniform U(-noise_amp, noise_amp)
@@ -133,6 +131,11 @@ int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, fl
  * Replaces the host FP64 numpy sums of Solver.monitors (lbm.py:649-661). */
 int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* params,
               const tlbm_fields* fields, double* sums, void* stream);
+
+/* A plain device-to-device copy kernel (16 bytes per lane; pointers and size 16-byte aligned).
+ * No counterpart in the reference: it is the known-size streaming kernel against which HBM
+ * counters and bandwidth of the step are calibrated. */
+int tlbm_copy(void* dst, const void* src, size_t bytes, void* stream);
 
 /* coarsen (lbm.py:516-522): dst[(k*my + j)*mx + i] = src[((k s)*ny + j s)*nx + i s]. */
 int tlbm_coarsen(const float* src, float* dst, int nx, int ny, int nz, int s, int mx, int my,

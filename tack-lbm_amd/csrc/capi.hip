@@ -45,10 +45,8 @@ int hip_fail(hipError_t e, const char* what) {
 constexpr int kHandles = 3 * 3 * 3 * 3 * 2;  // storage, collision, obstacle, side, diag
 
 tlbm::StepTable g_tables[3][3];
-// launch overrides per handle: threads per block (0 = default) and vector kernel use
-// (-1 = when the shape allows, 0 = never)
+// launch override per handle: threads per block along x (0 = default)
 int g_block_x[kHandles];
-int g_vector[kHandles];
 std::once_flag g_once;
 
 void build_tables() {
@@ -62,7 +60,6 @@ void build_tables() {
   tlbm_register_2_1(g_tables[2][1]);
   tlbm_register_2_2(g_tables[2][2]);
   for (int& b : g_block_x) b = 0;
-  for (int& v : g_vector) v = -1;
 }
 
 struct Decoded {
@@ -139,6 +136,13 @@ __global__ void coarsen_kernel(const float* __restrict__ src, float* __restrict_
   dst[i] = src[(static_cast<size_t>(c) * s * ny + static_cast<size_t>(b) * s) * nx + a * s];
 }
 
+// Plain device copy, 16 bytes per lane (the bandwidth calibration kernel of tlbm_copy).
+__global__ void __launch_bounds__(256) copy16_kernel(const uint4* __restrict__ src,
+                                                     uint4* __restrict__ dst, uint64_t n16) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n16) dst[i] = src[i];
+}
+
 int check_params(const tlbm_params* p) {
   if (!p) return fail(TLBM_ERR_INVALID, "params is NULL");
   if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0)
@@ -194,19 +198,19 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
   }
   a.u0 = p.u0;
   a.s0 = p.sponge_start;
-  a.s_len = static_cast<float>(p.nx) - p.sponge_start;
+  a.s_inv = 1.0f / (static_cast<float>(p.nx) - p.sponge_start);
   a.s_k = p.sponge_strength;
   for (int i = 0; i < 3; ++i) a.lid[i] = p.lid[i];
   a.omega_rot = p.omega_rot;
   return a;
 }
 
-// Threads per block along x for `threads_needed` threads per row: the smallest of 64/128/256
-// that covers the row, else 256.
-int default_block_x(int threads_needed) {
-  if (threads_needed <= 64) return 64;
-  if (threads_needed <= 128) return 128;
-  return 256;
+// Threads per block along x: the smallest of 64/128 that covers a row, else 256 for FP32 and
+// 128 for the 16-bit formats (the fastest shapes measured at 512 cells per row).
+int default_block_x(int nx, int storage) {
+  if (nx <= 64) return 64;
+  if (nx <= 128) return 128;
+  return storage == TLBM_F32 ? 256 : 128;
 }
 
 template <int O, int S>
@@ -381,16 +385,13 @@ int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle) {
   return TLBM_OK;
 }
 
-int tlbm_set_launch(int handle, int block_x, int vector) {
+int tlbm_set_launch(int handle, int block_x) {
   std::call_once(g_once, build_tables);
   Decoded d;
   if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
   if (block_x != 0 && block_x != 64 && block_x != 128 && block_x != 256)
     return fail(TLBM_ERR_INVALID, "block_x must be 0, 64, 128 or 256 (got %d)", block_x);
-  if (vector != -1 && vector != 0)
-    return fail(TLBM_ERR_INVALID, "vector must be -1 (auto) or 0 (off), got %d", vector);
   g_block_x[handle] = block_x;
-  g_vector[handle] = vector;
   return TLBM_OK;
 }
 
@@ -422,17 +423,9 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
   tlbm::StepArgs a = make_args(*params, d.obstacle);
   a.fin = f_in;
   a.fout = f_out;
-  // vector kernel: V cells per thread with aligned V-wide loads; needs nx and the plane stride
-  // to be multiples of V and 16-byte aligned buffers
-  const int vw = tlbm::vector_width(d.storage);
-  const bool vec = g_vector[handle] != 0 && params->nx % vw == 0 && params->qstride % vw == 0 &&
-                   reinterpret_cast<uintptr_t>(f_in) % 16 == 0 &&
-                   reinterpret_cast<uintptr_t>(f_out) % 16 == 0;
-  const int cells = vec ? vw : 1;
-  const int threads_per_row = (params->nx + cells - 1) / cells;
-  const int bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(threads_per_row);
+  const int bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(params->nx, d.storage);
   const dim3 block(bx);
-  const dim3 grid((threads_per_row + bx - 1) / bx, params->ny, params->nz);
+  const dim3 grid((params->nx + bx - 1) / bx, params->ny, params->nz);
   const uint32_t nblocks = grid.x * grid.y * grid.z;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (d.diag == TLBM_REPORT) {
@@ -448,8 +441,7 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
       a.nblocks = nblocks;
     }
   }
-  hipError_t e =
-      g_tables[d.storage][d.collision][d.obstacle][d.side][d.diag][vec ? 1 : 0](a, grid, block, st);
+  hipError_t e = g_tables[d.storage][d.collision][d.obstacle][d.side][d.diag](a, grid, block, st);
   if (e != hipSuccess) return hip_fail(e, "lbm_step launch");
   if (d.diag == TLBM_REPORT && sums) {
     hipLaunchKernelGGL(reduce_partials, dim3(TLBM_NSUMS), dim3(1024), 0, st, a.partials, nblocks,
@@ -457,6 +449,22 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "report reduction launch");
   }
+  return TLBM_OK;
+}
+
+int tlbm_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return TLBM_OK;
+  if (!dst || !src) return fail(TLBM_ERR_INVALID, "copy with NULL pointer");
+  if (bytes % 16 || reinterpret_cast<uintptr_t>(dst) % 16 || reinterpret_cast<uintptr_t>(src) % 16)
+    return fail(TLBM_ERR_INVALID, "copy needs 16-byte aligned pointers and size");
+  const uint64_t n16 = bytes / 16;
+  const uint64_t blocks = (n16 + 255) / 256;
+  if (blocks > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "copy too large");
+  hipLaunchKernelGGL(copy16_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const uint4*>(src),
+                     static_cast<uint4*>(dst), n16);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "copy launch");
   return TLBM_OK;
 }
 

@@ -1,17 +1,16 @@
 // Device code of the fused D3Q19 step for gfx950.
 //
-// Mapping.  Each thread owns V consecutive cells of one x-row; blocks are laid along x and y, z
-// come from blockIdx, so every y/z decision (periodic wrap, slip mirror, moving lid) is uniform
-// across the wave and every row/plane offset is a scalar.  Populations are loaded and stored as
-// aligned V-wide vectors (16 bytes for FP32 V=4 and for FP16/BF16 V=8): each population plane
-// is streamed in full 64-lane x 16-byte requests.  The +-1 x offsets of the pull scheme are
-// resolved in registers: a cell's x-1 (x+1) source is the previous (next) lane's last (first)
-// element, fetched with a lane shuffle; only the first (last) lane of a wave, or a lane at a
-// periodic wrap, issues one extra scalar load.  V = 1 is the plain one-cell-per-thread form used
-// for grids whose nx is not a multiple of the vector width.
+// Mapping.  One thread per cell; blocks are laid along x, and y and z come from blockIdx, so
+// every y/z decision (periodic wrap, slip mirror, moving lid) is uniform across the wave and
+// every row/plane offset is a scalar.  Each population plane is read and written in coalesced
+// wave-wide runs; the +-1 x offsets of the pull scheme are plain misaligned dword loads, served
+// by the same cache lines.  Measured on MI355X, this form streams as fast as 8/16-byte per-lane
+// vector loads with lane shuffles or two cells per thread, and runs the 16-bit storage formats
+// faster (fewer instructions, more waves per SIMD).  What matters most is that the 19 pull
+// loads of a thread are issued back to back, with no branch between them.
 //
-// Specialisation.  The kernel is a template over collision x obstacle x side x report x storage
-// (and V): each combination the reference's Tack compiler would specialise (lbm.py:1-7) is its
+// Specialisation.  The kernel is a template over collision x obstacle x side x report x storage:
+// each combination the reference's Tack compiler would specialise (lbm.py:1-7) is its
 // own instantiation, with no runtime branch on the configuration.
 //
 // Arithmetic follows lbm.py:397-513, rearranged only in ways that are exact or within FP32
@@ -79,30 +78,21 @@ __host__ __device__ __forceinline__ void static_for(F&& f) {
 struct StoreF32 {
   using T = float;
   static constexpr bool kShift = false;
-  static constexpr int kVec = 4;  // 16-byte vectors
   __device__ static __forceinline__ float value(T r) { return r; }
   __device__ static __forceinline__ T encode(float v) { return v; }
-  __device__ static __forceinline__ uint32_t bits(T r) { return __float_as_uint(r); }
-  __device__ static __forceinline__ T from_bits(uint32_t b) { return __uint_as_float(b); }
 };
 
 struct StoreF16 {
   using T = _Float16;
   static constexpr bool kShift = true;
-  static constexpr int kVec = 8;
   __device__ static __forceinline__ float value(T r) { return static_cast<float>(r); }
   __device__ static __forceinline__ T encode(float v) { return static_cast<T>(v); }
-  __device__ static __forceinline__ uint32_t bits(T r) { return __builtin_bit_cast(uint16_t, r); }
-  __device__ static __forceinline__ T from_bits(uint32_t b) {
-    return __builtin_bit_cast(T, static_cast<uint16_t>(b));
-  }
 };
 
 // bfloat16 = upper half of the binary32 pattern.
 struct StoreBF16 {
   using T = uint16_t;
   static constexpr bool kShift = true;
-  static constexpr int kVec = 8;
   __device__ static __forceinline__ float value(T r) {
     return __uint_as_float(static_cast<uint32_t>(r) << 16);
   }
@@ -110,13 +100,6 @@ struct StoreBF16 {
     const uint32_t u = __float_as_uint(v);
     return static_cast<T>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
   }
-  __device__ static __forceinline__ uint32_t bits(T r) { return r; }
-  __device__ static __forceinline__ T from_bits(uint32_t b) { return static_cast<T>(b); }
-};
-
-template <class T, int V>
-struct alignas(sizeof(T) * V) Pack {
-  T v[V];
 };
 
 // ---- kernel arguments (host fills them from tlbm_params) ----------------------------------
@@ -131,7 +114,7 @@ struct StepArgs {
   float o[6];                // obstacle parameters
   float r2;                  // r*r (cylinder, sphere)
   float reach[3];            // half extents + 2 (cells) for the neighbourhood bound
-  float u0, s0, s_len, s_k;  // sponge: start, nx - start, strength
+  float u0, s0, s_inv, s_k;  // sponge: start, 1 / (nx - start), strength
   float lid[3];
   const float* omega_rot;
   tlbm_fields fld;
@@ -240,9 +223,15 @@ __device__ __forceinline__ void equilibria(float rho, float ux, float uy, float 
   });
 }
 
+// Reciprocal and square root with single hardware instructions (v_rcp_f32 / v_sqrt_f32, within
+// 1 ulp) instead of the correctly rounded multi-instruction sequences: the step's FP32 error
+// budget against the FP64 oracle is ~1e-7 relative, far above one ulp of these factors.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // tau with the Smagorinsky eddy viscosity, lbm.py:72-93, from d = f - feq.  The stress only
-// involves the pair sums d_q + d_opp(q).
-__device__ __forceinline__ float smagorinsky_tau(const float (&d)[kQ], float rho, float tau0,
+// involves the pair sums d_q + d_opp(q).  inv_rho = 1 / rho.
+__device__ __forceinline__ float smagorinsky_tau(const float (&d)[kQ], float inv_rho, float tau0,
                                                  float smag) {
   float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pxz = 0.0f, pyz = 0.0f;
   static_for<0, 9>([&](auto pc) {
@@ -259,9 +248,9 @@ __device__ __forceinline__ float smagorinsky_tau(const float (&d)[kQ], float rho
     if constexpr (ey * ez == 1) pyz += s;
     if constexpr (ey * ez == -1) pyz -= s;
   });
-  const float pi = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz +
+  const float pi = fsqrt(pxx * pxx + pyy * pyy + pzz * pzz +
                          2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
-  return 0.5f * (tau0 + sqrtf(tau0 * tau0 + smag * pi / rho));
+  return 0.5f * (tau0 + fsqrt(tau0 * tau0 + smag * pi * inv_rho));
 }
 
 // ---- FP64 block reduction for the report variant -------------------------------------------
@@ -306,10 +295,11 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
   });
   if constexpr (SHIFT) rho += 1.0f;
   const float fluid = 1.0f - here;
-  const float inv_rho = fluid / rho;
-  const float ux = jx * inv_rho;
-  const float uy = jy * inv_rho;
-  const float uz = jz * inv_rho;
+  const float inv_rho = rcp(rho);
+  const float scale = fluid * inv_rho;
+  const float ux = jx * scale;
+  const float uy = jy * scale;
+  const float uz = jz * scale;
 
   // collision: d[q] = post_q - feq_q (lbm.py:60-139)
   float e[kQ];
@@ -323,13 +313,13 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
     const float k = 1.0f - a.omega;
     static_for<0, kQ>([&](auto qc) { d[decltype(qc)::value] *= k; });
   } else if constexpr (COLL == TLBM_LES) {
-    const float tau = smagorinsky_tau(d, rho, a.tau0, a.smag);
-    const float k = 1.0f - 1.0f / tau;
+    const float tau = smagorinsky_tau(d, inv_rho, a.tau0, a.smag);
+    const float k = 1.0f - rcp(tau);
     static_for<0, kQ>([&](auto qc) { d[decltype(qc)::value] *= k; });
   } else {
-    const float tau = smagorinsky_tau(d, rho, a.tau0, a.smag);
-    const float kp = 1.0f - 1.0f / tau;
-    const float km = 1.0f - 1.0f / (a.magic / (tau - 0.5f) + 0.5f);
+    const float tau = smagorinsky_tau(d, inv_rho, a.tau0, a.smag);
+    const float kp = 1.0f - rcp(tau);
+    const float km = 1.0f - rcp(a.magic * rcp(tau - 0.5f) + 0.5f);
     d[0] *= kp;
     static_for<0, 9>([&](auto pc) {
       constexpr int q = 2 * decltype(pc)::value + 1;
@@ -341,7 +331,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
   }
 
   // sponge toward the inflow state (lbm.py:446-456)
-  const float t = fmaxf(0.0f, (x - a.s0) / a.s_len);
+  const float t = fmaxf(0.0f, (x - a.s0) * a.s_inv);
   const float sigma = fmaxf(here, a.s_k * t * t);
   if (sigma != 0.0f) {
     const float rho_s = rho + sigma * (1.0f - rho);
@@ -368,7 +358,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
 }
 
 // ---- the fused step --------------------------------------------------------------------------
-template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, int V>
+template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE>
 __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   using Obs = ObstacleT<OBS>;
   using St = StorageT<STORAGE>;
@@ -378,23 +368,23 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   T* __restrict__ fout = static_cast<T*>(a.fout);
 
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int lane = threadIdx.x & 63;
-  const int x0raw = (blockIdx.x * blockDim.x + threadIdx.x) * V;
-  const int y = blockIdx.y;
-  const int z = blockIdx.z;
-  const bool active = x0raw < nx;
-  if constexpr (!REPORT && V == 1) {
+  const int xr = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = xr < nx;
+  if constexpr (!REPORT) {
     if (!active) return;
   }
-  // inactive lanes (past the end of the row) mirror the last cells so that every address stays
-  // in bounds and every lane takes part in the shuffles; they store nothing
-  const int x0 = active ? x0raw : nx - V;
+  // in the report variant every lane reaches the block reduction: lanes past the end of the
+  // row mirror the last cell (in-bounds loads) and store nothing
+  const int x = active ? xr : nx - 1;
+  const int y = blockIdx.y;
+  const int z = blockIdx.z;
   const uint32_t snx = static_cast<uint32_t>(nx);
   const uint32_t plane = snx * static_cast<uint32_t>(ny);
-  const uint32_t row0 = (static_cast<uint32_t>(z) * ny + y) * snx;
+  const uint32_t idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
 
-  // pull source rows (x, z periodic: lbm.py:412, 415; y by side: lbm.py:342-369), indexed
-  // [1 + c]: the source of a population moving with +c is one cell back.
+  // pull sources (x, z periodic: lbm.py:412, 415; y by side: lbm.py:342-369), indexed [1 + c]:
+  // the source of a population moving with +c is one cell back.
+  const int xsrc[3] = {x == nx - 1 ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};
   const int zsrc[3] = {z == nz - 1 ? 0 : z + 1, z, z == 0 ? nz - 1 : z - 1};
   int ysrc[3];
   bool mir_lo = false, mir_hi = false, lid_hi = false;
@@ -413,49 +403,19 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   ysrc[1] = y;
 
   // ---- pull streaming (lbm.py:410-426) ----
-  float f[V][kQ];
+  // No branch between the 19 loads, so that all of them are in flight together: ceiling links
+  // of the lid read their in-bounds (clamped) row and are overwritten below.
+  float f[kQ];
   static_for<0, kQ>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-    if constexpr (SIDE == TLBM_LID_Y && cy == -1) {
-      if (lid_hi) return;  // replaced by the lid bounce-back below
-    }
     int qs = q;
     if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = mir_lo ? miry(q) : q;
     if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = mir_hi ? miry(q) : q;
     const T* __restrict__ row = fin + static_cast<uint64_t>(qs) * a.ns +
                                 static_cast<uint32_t>(zsrc[1 + cz]) * plane +
                                 static_cast<uint32_t>(ysrc[1 + cy]) * snx;
-    if constexpr (V == 1) {
-      const int xs = cx == 1    ? (x0 == 0 ? nx - 1 : x0 - 1)
-                     : cx == -1 ? (x0 == nx - 1 ? 0 : x0 + 1)
-                                : x0;
-      f[0][q] = St::value(row[xs]);
-    } else {
-      const Pack<T, V> A = *reinterpret_cast<const Pack<T, V>*>(row + x0);
-      if constexpr (cx == 0) {
-#pragma unroll
-        for (int k = 0; k < V; ++k) f[k][q] = St::value(A.v[k]);
-      } else if constexpr (cx == 1) {
-        // x - 1 of the first cell: last element of the previous lane
-        const uint32_t b =
-            static_cast<uint32_t>(__shfl_up(static_cast<int>(St::bits(A.v[V - 1])), 1, 64));
-        T prev = St::from_bits(b);
-        if (lane == 0 || x0 == 0) prev = row[x0 == 0 ? nx - 1 : x0 - 1];
-        f[0][q] = St::value(prev);
-#pragma unroll
-        for (int k = 1; k < V; ++k) f[k][q] = St::value(A.v[k - 1]);
-      } else {
-        // x + 1 of the last cell: first element of the next lane
-        const uint32_t b =
-            static_cast<uint32_t>(__shfl_down(static_cast<int>(St::bits(A.v[0])), 1, 64));
-        T next = St::from_bits(b);
-        if (lane == 63 || x0 + V == nx) next = row[x0 + V == nx ? 0 : x0 + V];
-#pragma unroll
-        for (int k = 0; k < V - 1; ++k) f[k][q] = St::value(A.v[k + 1]);
-        f[V - 1][q] = St::value(next);
-      }
-    }
+    f[q] = St::value(row[xsrc[1 + cx]]);
   });
 
   // ---- moving lid: f_q = f_opp(x) + 6 w_q c_q . u_lid on ceiling links ----
@@ -467,77 +427,60 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
           constexpr int o = opp(q);
           const float t =
               6.0f * kW[q] * (kCX[q] * a.lid[0] + kCY[q] * a.lid[1] + kCZ[q] * a.lid[2]);
-          const Pack<T, V> B = *reinterpret_cast<const Pack<T, V>*>(
-              fin + static_cast<uint64_t>(o) * a.ns + row0 + x0);
-#pragma unroll
-          for (int k = 0; k < V; ++k) f[k][q] = St::value(B.v[k]) + t;
+          f[q] = St::value(fin[static_cast<uint64_t>(o) * a.ns + idx]) + t;
         }
       });
     }
   }
 
   // ---- obstacle: halfway / moving-wall bounce-back and momentum exchange (lbm.py:416-432) ----
-  float here[V], mx[V], my[V], mz[V];
-  bool wrap_yz = (a.wrap & 4) && (z == 0 || z == nz - 1);
-  if constexpr (SIDE == TLBM_PERIODIC_Y)
-    wrap_yz = wrap_yz || ((a.wrap & 2) && (y == 0 || y == ny - 1));
-#pragma unroll
-  for (int k = 0; k < V; ++k) {
-    here[k] = 0.0f;
-    mx[k] = my[k] = mz[k] = 0.0f;
-    const int x = x0 + k;
-    const bool wrap = wrap_yz || ((a.wrap & 1) && (x == 0 || x == nx - 1));
-    if (wrap || Obs::near(a, x, y, z)) {
-      here[k] = Obs::solid(a, x, y, z);
-      const float fluid = 1.0f - here[k];
-      float omega_rot = 0.0f;
-      if constexpr (Obs::kMoving) omega_rot = a.omega_rot ? *a.omega_rot : 0.0f;
-      const int xsrc[3] = {x == nx - 1 ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};
-      static_for<1, kQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-        constexpr int o = opp(q);
-        if constexpr (SIDE == TLBM_LID_Y && cy == -1) {
-          if (lid_hi) return;
+  float here = 0.0f;
+  float mx = 0.0f, my = 0.0f, mz = 0.0f;
+  bool wrap = ((a.wrap & 1) && (x == 0 || x == nx - 1)) ||
+              ((a.wrap & 4) && (z == 0 || z == nz - 1));
+  if constexpr (SIDE == TLBM_PERIODIC_Y) wrap = wrap || ((a.wrap & 2) && (y == 0 || y == ny - 1));
+  if (wrap || Obs::near(a, x, y, z)) {
+    here = Obs::solid(a, x, y, z);
+    const float fluid = 1.0f - here;
+    float omega_rot = 0.0f;
+    if constexpr (Obs::kMoving) omega_rot = a.omega_rot ? *a.omega_rot : 0.0f;
+    static_for<1, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+      constexpr int o = opp(q);
+      if constexpr (SIDE == TLBM_LID_Y && cy == -1) {
+        if (lid_hi) return;
+      }
+      const float wall = Obs::solid(a, xsrc[1 + cx], ysrc[1 + cy], zsrc[1 + cz]);
+      if (wall != 0.0f) {
+        const float b = St::value(fin[static_cast<uint64_t>(o) * a.ns + idx]);
+        float t = 0.0f;
+        if constexpr (Obs::kMoving) {
+          // wall velocity at the link midpoint (lbm.py:418-425)
+          float uwx, uwy, uwz;
+          Obs::wall_vel(a, static_cast<float>(x) - 0.5f * cx, static_cast<float>(y) - 0.5f * cy,
+                        static_cast<float>(z) - 0.5f * cz, uwx, uwy, uwz);
+          t = 6.0f * kW[q] *
+              (cx * (omega_rot * uwx) + cy * (omega_rot * uwy) + cz * (omega_rot * uwz));
         }
-        const float wall = Obs::solid(a, xsrc[1 + cx], ysrc[1 + cy], zsrc[1 + cz]);
-        if (wall != 0.0f) {
-          const float b = St::value(fin[static_cast<uint64_t>(o) * a.ns + row0 + x]);
-          float t = 0.0f;
-          if constexpr (Obs::kMoving) {
-            // wall velocity at the link midpoint (lbm.py:418-425)
-            float uwx, uwy, uwz;
-            Obs::wall_vel(a, static_cast<float>(x) - 0.5f * cx,
-                          static_cast<float>(y) - 0.5f * cy, static_cast<float>(z) - 0.5f * cz,
-                          uwx, uwy, uwz);
-            t = 6.0f * kW[q] *
-                (cx * (omega_rot * uwx) + cy * (omega_rot * uwy) + cz * (omega_rot * uwz));
-          }
-          f[k][q] = b + t;
-          if constexpr (REPORT) {
-            const float hit = fluid * (2.0f * (kShift ? b + kW[o] : b) + t);
-            mx[k] += hit * static_cast<float>(kCX[o]);
-            my[k] += hit * static_cast<float>(kCY[o]);
-            mz[k] += hit * static_cast<float>(kCZ[o]);
-          }
+        f[q] = b + t;
+        if constexpr (REPORT) {
+          const float hit = fluid * (2.0f * (kShift ? b + kW[o] : b) + t);
+          mx += hit * static_cast<float>(kCX[o]);
+          my += hit * static_cast<float>(kCY[o]);
+          mz += hit * static_cast<float>(kCZ[o]);
         }
-      });
-    }
+      }
+    });
   }
 
   // ---- collision, sponge, store ----
-  float rho[V], ux[V], uy[V], uz[V];
-#pragma unroll
-  for (int k = 0; k < V; ++k)
-    collide_cell<COLL, kShift>(a, f[k], static_cast<float>(x0 + k), here[k], rho[k], ux[k],
-                               uy[k], uz[k]);
+  float rho, ux, uy, uz;
+  collide_cell<COLL, kShift>(a, f, static_cast<float>(x), here, rho, ux, uy, uz);
   if (active) {
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      Pack<T, V> out;
-#pragma unroll
-      for (int k = 0; k < V; ++k) out.v[k] = St::encode(f[k][q]);
-      *reinterpret_cast<Pack<T, V>*>(fout + static_cast<uint64_t>(q) * a.ns + row0 + x0) = out;
+      fout[static_cast<uint64_t>(q) * a.ns + idx] = St::encode(f[q]);
     });
   }
 
@@ -548,38 +491,34 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
 #pragma unroll
     for (int i = 0; i < TLBM_NSUMS; ++i) acc[i] = 0.0;
     if (active) {
+      const float p = (rho - 1.0f) * (1.0f / 3.0f);
+      const float usq = ux * ux + uy * uy + uz * uz;
+      const float ke = 0.5f * rho * usq;
+      const float ox = F.ux[idx], oy = F.uy[idx], oz = F.uz[idx], op = F.p[idx];
+      const float ok = 0.5f * (ox * ox + oy * oy + oz * oz);
+      const float r_rho = (3.0f * (p - op)) * (3.0f * (p - op));
+      const float r_ux = (ux - ox) * (ux - ox);
+      const float r_uy = (uy - oy) * (uy - oy);
+      const float r_uz = (uz - oz) * (uz - oz);
+      const float r_ke = (ke - ok) * (ke - ok);
+      const float vmag = fsqrt(usq);
+      F.r_rho[idx] = r_rho;
+      F.r_ux[idx] = r_ux;
+      F.r_uy[idx] = r_uy;
+      F.r_uz[idx] = r_uz;
+      F.r_ke[idx] = r_ke;
+      F.ux[idx] = ux;
+      F.uy[idx] = uy;
+      F.uz[idx] = uz;
+      F.p[idx] = p;
+      F.vmag[idx] = vmag;
+      F.fx[idx] = mx;
+      F.fy[idx] = my;
+      F.fz[idx] = mz;
+      const float vals[TLBM_NSUMS] = {ux,   uy,   uz,   p,  vmag, r_rho, r_ux,
+                                      r_uy, r_uz, r_ke, mx, my,   mz,    rho};
 #pragma unroll
-      for (int k = 0; k < V; ++k) {
-        const uint32_t idx = row0 + x0 + k;
-        const float p = (rho[k] - 1.0f) / 3.0f;
-        const float usq = ux[k] * ux[k] + uy[k] * uy[k] + uz[k] * uz[k];
-        const float ke = 0.5f * rho[k] * usq;
-        const float ox = F.ux[idx], oy = F.uy[idx], oz = F.uz[idx], op = F.p[idx];
-        const float ok = 0.5f * (ox * ox + oy * oy + oz * oz);
-        const float r_rho = (3.0f * (p - op)) * (3.0f * (p - op));
-        const float r_ux = (ux[k] - ox) * (ux[k] - ox);
-        const float r_uy = (uy[k] - oy) * (uy[k] - oy);
-        const float r_uz = (uz[k] - oz) * (uz[k] - oz);
-        const float r_ke = (ke - ok) * (ke - ok);
-        const float vmag = sqrtf(usq);
-        F.r_rho[idx] = r_rho;
-        F.r_ux[idx] = r_ux;
-        F.r_uy[idx] = r_uy;
-        F.r_uz[idx] = r_uz;
-        F.r_ke[idx] = r_ke;
-        F.ux[idx] = ux[k];
-        F.uy[idx] = uy[k];
-        F.uz[idx] = uz[k];
-        F.p[idx] = p;
-        F.vmag[idx] = vmag;
-        F.fx[idx] = mx[k];
-        F.fy[idx] = my[k];
-        F.fz[idx] = mz[k];
-        const float vals[TLBM_NSUMS] = {ux[k], uy[k], uz[k], p,     vmag,  r_rho, r_ux,
-                                        r_uy,  r_uz,  r_ke,  mx[k], my[k], mz[k], rho[k]};
-#pragma unroll
-        for (int i = 0; i < TLBM_NSUMS; ++i) acc[i] += static_cast<double>(vals[i]);
-      }
+      for (int i = 0; i < TLBM_NSUMS; ++i) acc[i] = static_cast<double>(vals[i]);
     }
     if (a.partials) {
       const uint32_t bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);

@@ -8,34 +8,24 @@
 namespace tlbm {
 
 using StepLaunch = hipError_t (*)(const StepArgs&, dim3 grid, dim3 block, hipStream_t stream);
-// Launchers of one (storage, collision) unit, indexed [obstacle][side][report][vector]:
-// vector 0 is the one-cell-per-thread kernel, 1 the storage's vector width.
-using StepTable = StepLaunch[3][3][2][2];
+// Launchers of one (storage, collision) unit, indexed [obstacle][side][report].
+using StepTable = StepLaunch[3][3][2];
 
-template <int C, int O, int S, bool R, int ST, int V>
+template <int C, int O, int S, bool R, int ST>
 hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t stream) {
-  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, V>), grid, block, 0, stream, a);
+  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST>), grid, block, 0, stream, a);
   return hipGetLastError();
 }
 
 template <int C, int ST>
 void fill_table(StepTable& t) {
-  constexpr int kV = StorageT<ST>::kVec;
   static_for<0, 3>([&](auto o) {
     static_for<0, 3>([&](auto s) {
       constexpr int O = decltype(o)::value, S = decltype(s)::value;
-      t[O][S][0][0] = &launch_step<C, O, S, false, ST, 1>;
-      t[O][S][1][0] = &launch_step<C, O, S, true, ST, 1>;
-      t[O][S][0][1] = &launch_step<C, O, S, false, ST, kV>;
-      t[O][S][1][1] = &launch_step<C, O, S, true, ST, kV>;
+      t[O][S][0] = &launch_step<C, O, S, false, ST>;
+      t[O][S][1] = &launch_step<C, O, S, true, ST>;
     });
   });
-}
-
-// Vector width (cells per thread) of the vector kernels for a storage format.
-inline int vector_width(int storage) {
-  return storage == TLBM_F32 ? StoreF32::kVec : storage == TLBM_F16 ? StoreF16::kVec
-                                                                    : StoreBF16::kVec;
 }
 
 }  // namespace tlbm

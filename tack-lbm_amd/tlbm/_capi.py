@@ -67,11 +67,12 @@ SIGNATURES = {
     "tlbm_event_record": (_c.c_int, [_vp, _vp]),
     "tlbm_event_elapsed_ms": (_c.c_int, [_vp, _vp, _c.POINTER(_c.c_float)]),
     "tlbm_kernel_lookup": (_c.c_int, [_c.POINTER(Spec), _c.POINTER(_c.c_int)]),
-    "tlbm_set_launch": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int]),
+    "tlbm_set_launch": (_c.c_int, [_c.c_int, _c.c_int]),
     "tlbm_init": (_c.c_int, [_c.c_int, _vp, _c.POINTER(Params), _c.c_float, _c.c_float,
                              _c.c_uint64, _vp]),
     "tlbm_step": (_c.c_int, [_c.c_int, _vp, _vp, _c.POINTER(Params), _c.POINTER(Fields),
                              _vp, _vp]),
+    "tlbm_copy": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
     "tlbm_coarsen": (_c.c_int, [_vp, _vp, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int,
                                 _c.c_int, _c.c_int, _vp]),
 }

@@ -365,13 +365,12 @@ def kernel_handle(collision, obstacle, diag, side, storage):
     return h
 
 
-def set_launch(phys, geo, diag, side, storage, block_x=0, vector=-1):
-    """Launch shape of one specialisation (process-wide): threads per block along x (0 = the
-    default) and whether the vector kernel may be used (-1 = when the shape allows, 0 = never)."""
+def set_launch(phys, geo, diag, side, storage, block_x=0):
+    """Threads per block along x of one specialisation (process-wide; 0 = the default)."""
     h = kernel_handle(_code(phys, "collision"), _code(geo, "obstacle"),
                       _capi.REPORT if getattr(diag, "enabled", 0) == 1 else _capi.QUIET,
                       _code(side, "side"), STORAGES[storage])
-    check(lib().tlbm_set_launch(h, int(block_x), int(vector)), "set launch")
+    check(lib().tlbm_set_launch(h, int(block_x)), "set launch")
 
 
 def _code(obj, what):
@@ -553,10 +552,10 @@ class Solver:
                 check(lib().tlbm_memcpy_async(self.output[name].ptr, src.ptr, src.nbytes,
                                               self.stream), "output copy")
 
-    def set_launch(self, block_x=0, vector=-1):
-        """Launch shape of this solver's quiet and report kernels (see ``set_launch``)."""
+    def set_launch(self, block_x=0):
+        """Threads per block of this solver's quiet and report kernels (see ``set_launch``)."""
         for diag in (self.quiet, self.report):
-            set_launch(self.phys, self.geo, diag, self.flow.side, self.storage, block_x, vector)
+            set_launch(self.phys, self.geo, diag, self.flow.side, self.storage, block_x)
 
     def set_physics(self, collision, reynolds):
         """lbm.py:613-616: nu = u0 D / Re; a collision change selects another kernel."""

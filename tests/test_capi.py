@@ -106,8 +106,9 @@ def test_step_argument_errors_are_reported_before_any_launch():
     lib.tlbm_kernel_lookup(ctypes.byref(_capi.Spec(0, 0, 1, 0, 0)), ctypes.byref(rep))
     assert lib.tlbm_step(rep, fake_a, fake_b, ctypes.byref(p), None, None, None) != 0
     assert "report" in _capi.last_error()
-    assert lib.tlbm_set_launch(h, 96, -1) != 0
-    assert lib.tlbm_set_launch(h, 64, 2) != 0
+    assert lib.tlbm_set_launch(h, 96) != 0
+    assert lib.tlbm_set_launch(12345, 64) != 0
+    assert lib.tlbm_set_launch(h, 64) == 0 and lib.tlbm_set_launch(h, 0) == 0
 
 
 def test_shifted_population_codecs():

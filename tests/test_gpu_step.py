@@ -42,20 +42,18 @@ def test_trt_known_answer():
     assert _rel(got, d["f_out"]) < 1e-6
 
 
-def _run_step(case, f64, report, storage="f32", seed=3, vector=-1):
-    """One step on the GPU and in the oracle from the same populations (and old fields).
-
-    vector=-1 lets the library use the vector kernel when the shape allows; 0 forces the
-    one-cell-per-thread kernel."""
+def _run_step(case, f64, report, storage="f32", seed=3, block_x=0):
+    """One step on the GPU and in the oracle from the same populations (and old fields),
+    launched with `block_x` threads per block (0 = the library's default)."""
     import tlbm
     lat, phys, geo, flow = case.device()
     diag_kind = tlbm.Report({k: None for k in tlbm.FIELDS}) if report else \
         tlbm.Quiet({k: None for k in tlbm.FIELDS})
-    tlbm.set_launch(phys, geo, diag_kind, flow.side, storage, 0, vector)
+    tlbm.set_launch(phys, geo, diag_kind, flow.side, storage, block_x)
     try:
         return _run_step_inner(case, f64, report, storage, seed, lat, phys, geo, flow)
     finally:
-        tlbm.set_launch(phys, geo, diag_kind, flow.side, storage, 0, -1)
+        tlbm.set_launch(phys, geo, diag_kind, flow.side, storage, 0)
 
 
 def _run_step_inner(case, f64, report, storage, seed, lat, phys, geo, flow):
@@ -84,24 +82,25 @@ COMBOS = [(c, o, s) for c in ("bgk", "trt", "les") for o in ("cylinder", "sphere
           for s in ("periodic", "slip", "lid")]
 
 
-@pytest.mark.parametrize("vector", [-1, 0])
+@pytest.mark.parametrize("block_x", [0, 64])
 @pytest.mark.parametrize("collision,obstacle,side", COMBOS)
-def test_quiet_step_f32(collision, obstacle, side, vector):
+def test_quiet_step_f32(collision, obstacle, side, block_x):
     case = Case((24, 12, 9), collision, obstacle, side)
     f = random_populations(case.n, seed=11)
-    got, ref, *_ = _run_step(case, f, report=False, vector=vector)
+    got, ref, *_ = _run_step(case, f, report=False, block_x=block_x)
     assert _rel(got, ref) < 1e-6
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
 @pytest.mark.parametrize("dims", [(256, 3, 2), (512, 3, 2), (1024, 2, 2), (96, 4, 3), (16, 5, 4)])
 @pytest.mark.parametrize("side", ["periodic", "lid"])
-def test_vector_rows_across_waves(storage, dims, side):
-    """Rows of one, two and four waves (x-shift shuffles across wave edges and the periodic
-    wrap), and rows shorter than a wave; the obstacle straddles x = 0."""
+@pytest.mark.parametrize("block_x", [0, 64, 256])
+def test_rows_across_blocks(storage, dims, side, block_x):
+    """Rows of one to sixteen blocks (x streaming across block edges and the periodic wrap) and
+    rows shorter than a block; the obstacle straddles x = 0."""
     case = Case(dims, "les", "sphere", side, geometry=(1.3, 1.4, 1.2, 2.2))
     f = random_populations(case.n, seed=21)
-    got, ref, *_ = _run_step(case, f, report=True, storage=storage)
+    got, ref, *_ = _run_step(case, f, report=True, storage=storage, block_x=block_x)
     if storage == "f32":
         assert _rel(got, ref) < 1e-6
     else:
@@ -111,12 +110,12 @@ def test_vector_rows_across_waves(storage, dims, side):
         assert bool(np.all(np.abs(got - ref) <= bound))
 
 
-@pytest.mark.parametrize("vector", [-1, 0])
+@pytest.mark.parametrize("block_x", [0, 64])
 @pytest.mark.parametrize("collision,obstacle,side", COMBOS)
-def test_report_step_f32(collision, obstacle, side, vector):
+def test_report_step_f32(collision, obstacle, side, block_x):
     case = Case((24, 12, 9), collision, obstacle, side)
     f = random_populations(case.n, seed=12)
-    got, ref, g, o, sums = _run_step(case, f, report=True, vector=vector)
+    got, ref, g, o, sums = _run_step(case, f, report=True, block_x=block_x)
     assert _rel(got, ref) < 1e-6
     for k in O.FIELDS:
         scale = max(float(np.abs(o[k]).max()), 1e-12)
@@ -150,9 +149,8 @@ def test_ragged_and_degenerate_grids(dims):
 @pytest.mark.parametrize("storage,bits", [("f16", 10), ("bf16", 7)])
 @pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
 @pytest.mark.parametrize("side", ["periodic", "slip", "lid"])
-@pytest.mark.parametrize("nx", [24, 32])
+@pytest.mark.parametrize("nx", [24, 130])
 def test_shifted_storage(storage, bits, collision, side, nx):
-    """nx = 24 runs the one-cell kernel (not a multiple of 8), nx = 32 the vector kernel."""
     case = Case((nx, 12, 9), collision, "cylinder", side)
     f = random_populations(case.n, seed=15)
     got, ref, *_ = _run_step(case, f, report=False, storage=storage)
