@@ -147,8 +147,6 @@ int check_params(const tlbm_params* p) {
   if (!p) return fail(TLBM_ERR_INVALID, "params is NULL");
   if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0)
     return fail(TLBM_ERR_INVALID, "grid %dx%dx%d must be positive", p->nx, p->ny, p->nz);
-  if (p->ny > 65535 || p->nz > 65535)
-    return fail(TLBM_ERR_INVALID, "ny and nz must be <= 65535 (got %d, %d)", p->ny, p->nz);
   const uint64_t n = static_cast<uint64_t>(p->nx) * p->ny * p->nz;
   if (n > 0xFFFFFFFFull)
     return fail(TLBM_ERR_INVALID, "%llu cells exceed the 32-bit cell index",
@@ -404,8 +402,14 @@ int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, fl
   if (int rc = check_params(params)) return rc;
   tlbm::StepArgs a = make_args(*params, d.obstacle);
   a.fout = f;
+  const uint64_t xblocks = (params->nx + 255) / 256;
+  const uint64_t rows = static_cast<uint64_t>(params->ny) * params->nz;
+  if (xblocks * rows * 256 > 0xFFFFFFFFull)
+    return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
+  a.xblocks = static_cast<uint32_t>(xblocks);
+  a.rows = static_cast<uint32_t>(rows);
   const dim3 block(256);
-  const dim3 grid((params->nx + 255) / 256, params->ny, params->nz);
+  const dim3 grid(static_cast<uint32_t>(xblocks * rows));
   hipError_t e = kInit[d.storage][d.obstacle](a, seed_amp, noise_amp, noise_seed, grid, block,
                                               static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "lbm_init launch");
@@ -424,8 +428,15 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
   a.fin = f_in;
   a.fout = f_out;
   const int bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(params->nx, d.storage);
+  const uint64_t xblocks = (params->nx + bx - 1) / bx;
+  const uint64_t rows = static_cast<uint64_t>(params->ny) * params->nz;
+  if (xblocks * rows * bx > 0xFFFFFFFFull)
+    return fail(TLBM_ERR_INVALID, "grid of %llu threads exceeds the launch limit",
+                static_cast<unsigned long long>(xblocks * rows * bx));
+  a.xblocks = static_cast<uint32_t>(xblocks);
+  a.rows = static_cast<uint32_t>(rows);
   const dim3 block(bx);
-  const dim3 grid((params->nx + bx - 1) / bx, params->ny, params->nz);
+  const dim3 grid(static_cast<uint32_t>(xblocks * rows));
   const uint32_t nblocks = grid.x * grid.y * grid.z;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (d.diag == TLBM_REPORT) {

@@ -1,8 +1,8 @@
 // Device code of the fused D3Q19 step for gfx950.
 //
-// Mapping.  One thread per cell; blocks are laid along x, and y and z come from blockIdx, so
-// every y/z decision (periodic wrap, slip mirror, moving lid) is uniform across the wave and
-// every row/plane offset is a scalar.  Each population plane is read and written in coalesced
+// Mapping.  One thread per cell; each block covers part of one x-row (tile_of_block maps block
+// ids to rows XCD by XCD), so every y/z decision (periodic wrap, slip mirror, moving lid) is
+// uniform across the wave and every row/plane offset is a scalar.  Each population plane is read and written in coalesced
 // wave-wide runs; the +-1 x offsets of the pull scheme are plain misaligned dword loads, served
 // by the same cache lines.  Measured on MI355X, this form streams as fast as 8/16-byte per-lane
 // vector loads with lane shuffles or two cells per thread, and runs the 16-bit storage formats
@@ -120,6 +120,7 @@ struct StepArgs {
   tlbm_fields fld;
   double* partials;          // report: per-block FP64 partial sums [TLBM_NSUMS][nblocks]
   uint32_t nblocks;
+  uint32_t xblocks, rows;    // launch shape: blocks per x-row, ny * nz rows
 };
 
 // ---- obstacles: lbm.py:144-227 ---------------------------------------------------------------
@@ -357,6 +358,30 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
   uz_out = uz;
 }
 
+// ---- XCD-aware block -> tile mapping -----------------------------------------------------------
+// The grid is one-dimensional: xblocks blocks per x-row, rows = ny * nz rows.  Workgroups are
+// dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8), and each XCD has its own
+// L2.  The x-blocks of one row share a cache line at every block edge (the x +- 1 pulls), so
+// the blocks of a row are placed on one XCD: in each group of 8 * xblocks consecutive block
+// ids, XCD j holds all x-blocks of the group's j-th row.  Rows past the last whole group
+// (rows % 8 of them) take the plain order.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ void tile_of_block(uint32_t b, uint32_t xblocks, uint32_t rows,
+                                              uint32_t& xb, uint32_t& row) {
+  const uint32_t group = kXcds * xblocks;
+  const uint32_t whole = (rows / kXcds) * group;
+  if (b < whole) {
+    const uint32_t g = b / group, r = b % group;
+    row = g * kXcds + r % kXcds;
+    xb = r / kXcds;
+  } else {
+    const uint32_t r = b - whole;
+    row = (rows / kXcds) * kXcds + r / xblocks;
+    xb = r % xblocks;
+  }
+}
+
 // ---- the fused step --------------------------------------------------------------------------
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE>
 __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
@@ -368,7 +393,9 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   T* __restrict__ fout = static_cast<T*>(a.fout);
 
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int xr = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t xb, row;
+  tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
+  const int xr = static_cast<int>(xb * blockDim.x + threadIdx.x);
   const bool active = xr < nx;
   if constexpr (!REPORT) {
     if (!active) return;
@@ -376,8 +403,8 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   // in the report variant every lane reaches the block reduction: lanes past the end of the
   // row mirror the last cell (in-bounds loads) and store nothing
   const int x = active ? xr : nx - 1;
-  const int y = blockIdx.y;
-  const int z = blockIdx.z;
+  const int y = static_cast<int>(row % static_cast<uint32_t>(ny));
+  const int z = static_cast<int>(row / static_cast<uint32_t>(ny));
   const uint32_t snx = static_cast<uint32_t>(nx);
   const uint32_t plane = snx * static_cast<uint32_t>(ny);
   const uint32_t idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
@@ -520,10 +547,7 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
 #pragma unroll
       for (int i = 0; i < TLBM_NSUMS; ++i) acc[i] = static_cast<double>(vals[i]);
     }
-    if (a.partials) {
-      const uint32_t bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-      block_sums(acc, a.partials, a.nblocks, bid);
-    }
+    if (a.partials) block_sums(acc, a.partials, a.nblocks, blockIdx.x);
   }
 }
 
@@ -545,8 +569,11 @@ __global__ void __launch_bounds__(256) lbm_init_kernel(const StepArgs a, float s
   using St = StorageT<STORAGE>;
   using T = typename St::T;
   T* __restrict__ f = static_cast<T*>(a.fout);
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
+  uint32_t xb, row;
+  tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
+  const int x = static_cast<int>(xb * blockDim.x + threadIdx.x);
+  const int y = static_cast<int>(row % static_cast<uint32_t>(a.ny));
+  const int z = static_cast<int>(row / static_cast<uint32_t>(a.ny));
   if (x >= a.nx) return;
   const uint32_t idx = (static_cast<uint32_t>(z) * a.ny + y) * static_cast<uint32_t>(a.nx) + x;
   const float fluid = 1.0f - Obs::solid(a, x, y, z);
