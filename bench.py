@@ -127,6 +127,24 @@ def time_gpu(storage, steps, warmup, block_x=0):
     return out
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
+                               "lbm_step_les_sphere_lid_512x256x256_hbm_traffic.txt")
+
+
+def profiled_traffic(storage):
+    """HBM bytes per quiet launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated on a
+    copy of known size) and committed under profiles/; None if not measured for `storage`."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == storage:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def calibrate_copy(gib=1, reps=5):
     """A plain copy of `gib` GiB (read + write) with the library's copy kernel: the known-size
     reference for HBM counters and the achievable streaming bandwidth on this device."""
@@ -216,12 +234,15 @@ def main():
                    "cells": NX * NY * NZ, "storage": head["storage"]},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1),
                      "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": round(head["achieved_gbs"] / PEAK_GBS, 4), "traffic": None},
+                     "frac": round(head["achieved_gbs"] / PEAK_GBS, 4),
+                     "traffic": profiled_traffic(head["storage"]),
+                     "algorithmic_bytes": BYTES_PER_CELL[head["storage"]] * NX * NY * NZ},
         "cpu_baseline": cpu,
         "variants": {st: {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
                           "quiet_kernel_ms": round(r["quiet_kernel_ms"], 4),
                           "achieved_gbs": round(r["achieved_gbs"], 1),
                           "roofline_frac": round(r["achieved_gbs"] / PEAK_GBS, 4),
+                          "traffic": profiled_traffic(st),
                           "drag": r["monitors"]["drag"], "lift": r["monitors"]["lift"]}
                      for st, r in results.items()},
     }
