@@ -68,9 +68,11 @@ def _c1_reference():
     return _C1_REF["r"]
 
 
-# bounds per storage: (max |du| / u0, rms |du| / u0, max |d drag| / mean drag, max |d lift| / drag)
-C1_BOUNDS = {"f32": (2e-5, 2e-6, 1e-5, 1e-5), "f16": (5e-3, 5e-4, 5e-3, 5e-3),
-             "bf16": (5e-2, 5e-3, 5e-2, 5e-2)}
+# bounds per storage: (max |du| / u0, rms |du| / u0, max |d drag| / mean drag, max |d lift| /
+# mean drag), about 3x the errors measured on MI355X (f32 3.4e-5, 1.2e-5, 1.7e-5, 2.4e-6;
+# f16 1.3e-3, 3.0e-4, 5.0e-4, 3.2e-4; bf16 1.1e-2, 2.8e-3, 8.3e-3, 2.1e-3)
+C1_BOUNDS = {"f32": (1e-4, 4e-5, 6e-5, 1e-5), "f16": (4e-3, 1e-3, 2e-3, 1e-3),
+             "bf16": (4e-2, 1e-2, 2.5e-2, 7e-3)}
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
