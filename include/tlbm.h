@@ -141,6 +141,47 @@ int tlbm_copy(void* dst, const void* src, size_t bytes, void* stream);
 int tlbm_coarsen(const float* src, float* dst, int nx, int ny, int nz, int s, int mx, int my,
                  int mz, void* stream);
 
+/* ---- batched lattices (parameter sweeps) --------------------------------------------------
+ * No counterpart in the reference, which steps one lattice per launch.  A batch is a set of
+ * independent lattices that share one specialisation (collision, obstacle, side, storage of
+ * `handle`; its diag is ignored, both variants are available) and differ in size and runtime
+ * arguments (params[d]; qstride is ignored).  Their populations live in one buffer: domain d at
+ * element offset elem_offset(d), q-major with plane stride nx*ny*nz of that domain; packed
+ * per-cell arrays (report fields) hold domain d at cell_offset(d).  One launch steps every
+ * domain: a persistent grid walks chunks of block_x consecutive cells of one domain. */
+typedef struct tlbm_batch tlbm_batch;
+typedef struct tlbm_graph tlbm_graph;
+
+int tlbm_batch_create(int handle, const tlbm_params* params, int ndomains, tlbm_batch** out);
+int tlbm_batch_destroy(tlbm_batch* batch);
+/* Offsets of `domain` (domain == ndomains gives the totals) and the buffer sizes: `elements`
+ * population elements, `cells` lattice cells over all domains.  Any output may be NULL. */
+int tlbm_batch_layout(const tlbm_batch* batch, int domain, uint64_t* elem_offset,
+                      uint64_t* cell_offset, uint64_t* elements, uint64_t* cells);
+/* Threads per block (64, 128, 256; 0 = 256) and persistent grid size in blocks (0 = one block
+ * per chunk).  Changing block_x synchronises the device and rebuilds the chunk tables. */
+int tlbm_batch_set_launch(tlbm_batch* batch, int block_x, int grid_blocks);
+/* lbm_init of every domain (see tlbm_init). */
+int tlbm_batch_init(tlbm_batch* batch, void* f, float seed_amp, float noise_amp,
+                    uint64_t noise_seed, void* stream);
+/* One step of every domain.  report != 0: the report variant; `fields` (packed over all domains)
+ * may be NULL (then only the sums of u, p, |u|, forces and mass are formed and the residual
+ * sums are 0); `sums`, when not NULL, receives TLBM_NSUMS FP64 totals per domain
+ * (sums[d*TLBM_NSUMS + k]). */
+int tlbm_batch_step(tlbm_batch* batch, const void* f_in, void* f_out, int report,
+                    const tlbm_fields* fields, double* sums, void* stream);
+/* Capture `steps` (even) consecutive batch steps, f_a -> f_b -> f_a ..., with a report step
+ * every `report_every` steps (0: none), on `stream` (not the null stream) into a hipGraph;
+ * launching the graph runs them all and leaves the state in f_a. */
+int tlbm_batch_graph(tlbm_batch* batch, void* f_a, void* f_b, int steps, int report_every,
+                     const tlbm_fields* fields, double* sums, void* stream, tlbm_graph** out);
+int tlbm_graph_launch(tlbm_graph* graph, void* stream);
+int tlbm_graph_destroy(tlbm_graph* graph);
+/* Compute units and architecture name of the current device (sizing persistent grids, keying
+ * tuned launch shapes). */
+int tlbm_device_cus(int* cus);
+int tlbm_device_name(char* buf, int len);
+
 #ifdef __cplusplus
 }
 #endif

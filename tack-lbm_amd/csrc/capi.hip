@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -45,20 +46,21 @@ int hip_fail(hipError_t e, const char* what) {
 constexpr int kHandles = 3 * 3 * 3 * 3 * 2;  // storage, collision, obstacle, side, diag
 
 tlbm::StepTable g_tables[3][3];
+tlbm::BatchTable g_batch_tables[3][3];
 // launch override per handle: threads per block along x (0 = default)
 int g_block_x[kHandles];
 std::once_flag g_once;
 
 void build_tables() {
-  tlbm_register_0_0(g_tables[0][0]);
-  tlbm_register_0_1(g_tables[0][1]);
-  tlbm_register_0_2(g_tables[0][2]);
-  tlbm_register_1_0(g_tables[1][0]);
-  tlbm_register_1_1(g_tables[1][1]);
-  tlbm_register_1_2(g_tables[1][2]);
-  tlbm_register_2_0(g_tables[2][0]);
-  tlbm_register_2_1(g_tables[2][1]);
-  tlbm_register_2_2(g_tables[2][2]);
+  tlbm_register_0_0(g_tables[0][0], g_batch_tables[0][0]);
+  tlbm_register_0_1(g_tables[0][1], g_batch_tables[0][1]);
+  tlbm_register_0_2(g_tables[0][2], g_batch_tables[0][2]);
+  tlbm_register_1_0(g_tables[1][0], g_batch_tables[1][0]);
+  tlbm_register_1_1(g_tables[1][1], g_batch_tables[1][1]);
+  tlbm_register_1_2(g_tables[1][2], g_batch_tables[1][2]);
+  tlbm_register_2_0(g_tables[2][0], g_batch_tables[2][0]);
+  tlbm_register_2_1(g_tables[2][1], g_batch_tables[2][1]);
+  tlbm_register_2_2(g_tables[2][2], g_batch_tables[2][2]);
   for (int& b : g_block_x) b = 0;
 }
 
@@ -224,6 +226,112 @@ const InitLaunch kInit[3][3] = {
     {&launch_init<0, 0>, &launch_init<1, 0>, &launch_init<2, 0>},
     {&launch_init<0, 1>, &launch_init<1, 1>, &launch_init<2, 1>},
     {&launch_init<0, 2>, &launch_init<1, 2>, &launch_init<2, 2>}};  // [storage][obstacle]
+
+// Per-domain totals of a batched report step: block (d, k) sums partials[k][chunk] over the
+// chunks of domain d in a fixed order.
+__global__ void __launch_bounds__(256) reduce_domains(const double* __restrict__ partials,
+                                                      uint32_t nchunks,
+                                                      const uint32_t* __restrict__ chunk_off,
+                                                      double* __restrict__ out) {
+  __shared__ double red[256];
+  const int d = blockIdx.x, k = blockIdx.y;
+  const uint32_t c0 = chunk_off[d], c1 = chunk_off[d + 1];
+  double s = 0.0;
+  for (uint32_t c = c0 + threadIdx.x; c < c1; c += blockDim.x)
+    s += partials[static_cast<size_t>(k) * nchunks + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[static_cast<size_t>(d) * TLBM_NSUMS + k] = red[0];
+}
+
+}  // namespace
+
+// A batch of independent lattices stepped by one launch (see include/tlbm.h).
+struct tlbm_batch {
+  int handle = -1;  // quiet handle of the specialisation (report = handle + 1)
+  Decoded spec{};
+  int device = 0;
+  std::vector<tlbm_params> params;
+  std::vector<tlbm::BatchDomain> dom;
+  std::vector<uint32_t> chunk_off;  // ndom + 1
+  uint64_t elements = 0, cells = 0;
+  uint32_t nchunks = 0;
+  int block = 256;
+  int grid = 0;  // persistent grid (blocks); 0 = one block per chunk
+  tlbm::BatchDomain* d_dom = nullptr;
+  uint16_t* d_chunk_dom = nullptr;
+  uint32_t* d_chunk_off = nullptr;
+  double* d_partials = nullptr;
+};
+
+struct tlbm_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void batch_free_tables(tlbm_batch* b) {
+  if (b->d_dom) (void)hipFree(b->d_dom);
+  if (b->d_chunk_dom) (void)hipFree(b->d_chunk_dom);
+  if (b->d_chunk_off) (void)hipFree(b->d_chunk_off);
+  if (b->d_partials) (void)hipFree(b->d_partials);
+  b->d_dom = nullptr;
+  b->d_chunk_dom = nullptr;
+  b->d_chunk_off = nullptr;
+  b->d_partials = nullptr;
+}
+
+// (Re)build the chunk tables for the current block size and upload them.
+int batch_build(tlbm_batch* b) {
+  const int ndom = static_cast<int>(b->params.size());
+  b->dom.assign(ndom, tlbm::BatchDomain{});
+  b->chunk_off.assign(ndom + 1, 0);
+  uint64_t elem = 0, cell = 0, chunk = 0;
+  for (int d = 0; d < ndom; ++d) {
+    const tlbm_params& p = b->params[d];
+    const uint64_t n = static_cast<uint64_t>(p.nx) * p.ny * p.nz;
+    tlbm::BatchDomain& D = b->dom[d];
+    D.a = make_args(p, b->spec.obstacle);
+    D.a.ns = n;
+    D.elem_off = elem;
+    D.cell_off = static_cast<uint32_t>(cell);
+    D.chunk_off = static_cast<uint32_t>(chunk);
+    D.inv_nx = 1.0f / static_cast<float>(p.nx);
+    D.inv_nxny = 1.0f / static_cast<float>(static_cast<uint64_t>(p.nx) * p.ny);
+    b->chunk_off[d] = static_cast<uint32_t>(chunk);
+    elem += 19 * n;
+    cell += n;
+    chunk += (n + b->block - 1) / b->block;
+  }
+  if (cell > 0xFFFFFFFFull || chunk > 0xFFFFFFFFull)
+    return fail(TLBM_ERR_INVALID, "batch of %llu cells is too large",
+                static_cast<unsigned long long>(cell));
+  b->chunk_off[ndom] = static_cast<uint32_t>(chunk);
+  b->elements = elem;
+  b->cells = cell;
+  b->nchunks = static_cast<uint32_t>(chunk);
+  std::vector<uint16_t> chunk_dom(b->nchunks);
+  for (int d = 0; d < ndom; ++d)
+    for (uint32_t c = b->chunk_off[d]; c < b->chunk_off[d + 1]; ++c)
+      chunk_dom[c] = static_cast<uint16_t>(d);
+  batch_free_tables(b);
+  TLBM_HIP(hipMalloc(&b->d_dom, sizeof(tlbm::BatchDomain) * ndom));
+  TLBM_HIP(hipMalloc(&b->d_chunk_dom, sizeof(uint16_t) * b->nchunks));
+  TLBM_HIP(hipMalloc(&b->d_chunk_off, sizeof(uint32_t) * (ndom + 1)));
+  TLBM_HIP(hipMalloc(&b->d_partials, sizeof(double) * TLBM_NSUMS * b->nchunks));
+  TLBM_HIP(hipMemcpy(b->d_dom, b->dom.data(), sizeof(tlbm::BatchDomain) * ndom,
+                     hipMemcpyHostToDevice));
+  TLBM_HIP(hipMemcpy(b->d_chunk_dom, chunk_dom.data(), sizeof(uint16_t) * b->nchunks,
+                     hipMemcpyHostToDevice));
+  TLBM_HIP(hipMemcpy(b->d_chunk_off, b->chunk_off.data(), sizeof(uint32_t) * (ndom + 1),
+                     hipMemcpyHostToDevice));
+  return TLBM_OK;
+}
 
 }  // namespace
 
@@ -495,6 +603,194 @@ int tlbm_coarsen(const float* src, float* dst, int nx, int ny, int nz, int s, in
                      src, dst, nx, ny, s, mx, my, mz);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "coarsen launch");
+  return TLBM_OK;
+}
+
+int tlbm_batch_create(int handle, const tlbm_params* params, int ndomains, tlbm_batch** out) {
+  std::call_once(g_once, build_tables);
+  if (!out) return fail(TLBM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  Decoded d;
+  if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
+  if (!params || ndomains <= 0 || ndomains > 65535)
+    return fail(TLBM_ERR_INVALID, "need 1..65535 domains (got %d)", ndomains);
+  for (int i = 0; i < ndomains; ++i) {
+    tlbm_params p = params[i];
+    p.qstride = static_cast<int64_t>(p.nx) * p.ny * p.nz;
+    if (int rc = check_params(&p)) return rc;
+    if (p.qstride >= (1 << 24))
+      return fail(TLBM_ERR_INVALID, "domain %d has %lld cells; batched domains hold < 2^24", i,
+                  static_cast<long long>(p.qstride));
+  }
+  tlbm_batch* b = new tlbm_batch();
+  b->handle = handle - d.diag;
+  b->spec = d;
+  (void)hipGetDevice(&b->device);
+  b->params.assign(params, params + ndomains);
+  for (auto& p : b->params) p.qstride = static_cast<int64_t>(p.nx) * p.ny * p.nz;
+  if (int rc = batch_build(b)) {
+    batch_free_tables(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return TLBM_OK;
+}
+
+int tlbm_batch_destroy(tlbm_batch* b) {
+  if (!b) return TLBM_OK;
+  batch_free_tables(b);
+  delete b;
+  return TLBM_OK;
+}
+
+int tlbm_batch_layout(const tlbm_batch* b, int domain, uint64_t* elem_offset,
+                      uint64_t* cell_offset, uint64_t* elements, uint64_t* cells) {
+  if (!b) return fail(TLBM_ERR_INVALID, "batch is NULL");
+  const int ndom = static_cast<int>(b->params.size());
+  if (domain < 0 || domain > ndom)
+    return fail(TLBM_ERR_INVALID, "domain %d outside 0..%d", domain, ndom);
+  if (elem_offset) *elem_offset = domain < ndom ? b->dom[domain].elem_off : b->elements;
+  if (cell_offset) *cell_offset = domain < ndom ? b->dom[domain].cell_off : b->cells;
+  if (elements) *elements = b->elements;
+  if (cells) *cells = b->cells;
+  return TLBM_OK;
+}
+
+int tlbm_batch_set_launch(tlbm_batch* b, int block_x, int grid_blocks) {
+  if (!b) return fail(TLBM_ERR_INVALID, "batch is NULL");
+  if (block_x == 0) block_x = 256;
+  if (block_x != 64 && block_x != 128 && block_x != 256)
+    return fail(TLBM_ERR_INVALID, "block_x must be 0, 64, 128 or 256 (got %d)", block_x);
+  if (grid_blocks < 0) return fail(TLBM_ERR_INVALID, "grid_blocks must be >= 0");
+  b->grid = grid_blocks;
+  if (block_x != b->block) {
+    TLBM_HIP(hipDeviceSynchronize());  // tables may be in use by queued work
+    b->block = block_x;
+    return batch_build(b);
+  }
+  return TLBM_OK;
+}
+
+int tlbm_batch_init(tlbm_batch* b, void* f, float seed_amp, float noise_amp, uint64_t noise_seed,
+                    void* stream) {
+  if (!b || !f) return fail(TLBM_ERR_INVALID, "batch or f is NULL");
+  for (size_t i = 0; i < b->params.size(); ++i) {
+    const tlbm_params& p = b->params[i];
+    tlbm::StepArgs a = b->dom[i].a;
+    a.fout = static_cast<char*>(f) + b->dom[i].elem_off * (b->spec.storage == TLBM_F32 ? 4 : 2);
+    const uint64_t xblocks = (p.nx + 255) / 256;
+    a.xblocks = static_cast<uint32_t>(xblocks);
+    a.rows = static_cast<uint32_t>(static_cast<uint64_t>(p.ny) * p.nz);
+    const dim3 grid(static_cast<uint32_t>(xblocks * a.rows));
+    hipError_t e = kInit[b->spec.storage][b->spec.obstacle](a, seed_amp, noise_amp, noise_seed,
+                                                           grid, dim3(256),
+                                                           static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "batch init launch");
+  }
+  return TLBM_OK;
+}
+
+int tlbm_batch_step(tlbm_batch* b, const void* f_in, void* f_out, int report,
+                    const tlbm_fields* fields, double* sums, void* stream) {
+  if (!b || !f_in || !f_out) return fail(TLBM_ERR_INVALID, "batch, f_in or f_out is NULL");
+  if (f_in == f_out) return fail(TLBM_ERR_INVALID, "f_in and f_out must be distinct buffers");
+  tlbm::BatchArgs a{};
+  a.fin = f_in;
+  a.fout = f_out;
+  a.dom = b->d_dom;
+  a.chunk_dom = b->d_chunk_dom;
+  a.ndom = static_cast<int32_t>(b->params.size());
+  a.nchunks = b->nchunks;
+  if (report) {
+    if (fields) {
+      const float* const* ptrs = reinterpret_cast<const float* const*>(fields);
+      for (int k = 0; k < 13; ++k)
+        if (!ptrs[k]) return fail(TLBM_ERR_INVALID, "report field %d is NULL", k);
+      a.fld = *fields;
+    }
+    a.partials = sums ? b->d_partials : nullptr;
+  }
+  const uint32_t grid = b->grid > 0 ? std::min<uint32_t>(b->grid, b->nchunks) : b->nchunks;
+  const Decoded& d = b->spec;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipError_t e = g_batch_tables[d.storage][d.collision][d.obstacle][d.side][report ? 1 : 0](
+      a, dim3(grid), dim3(b->block), st);
+  if (e != hipSuccess) return hip_fail(e, "batch step launch");
+  if (report && sums) {
+    hipLaunchKernelGGL(reduce_domains, dim3(static_cast<uint32_t>(b->params.size()), TLBM_NSUMS),
+                       dim3(256), 0, st, b->d_partials, b->nchunks, b->d_chunk_off, sums);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "batch reduction launch");
+  }
+  return TLBM_OK;
+}
+
+int tlbm_batch_graph(tlbm_batch* b, void* f_a, void* f_b, int steps, int report_every,
+                     const tlbm_fields* fields, double* sums, void* stream, tlbm_graph** out) {
+  if (!out) return fail(TLBM_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (!b || !f_a || !f_b) return fail(TLBM_ERR_INVALID, "batch or buffers are NULL");
+  if (!stream) return fail(TLBM_ERR_INVALID, "graph capture needs a created (non-null) stream");
+  if (steps <= 0 || steps % 2) return fail(TLBM_ERR_INVALID, "steps must be even and > 0");
+  if (report_every < 0) return fail(TLBM_ERR_INVALID, "report_every must be >= 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  TLBM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = TLBM_OK;
+  for (int i = 0; i < steps && rc == TLBM_OK; ++i) {
+    const bool report = report_every > 0 && (i + 1) % report_every == 0;
+    rc = tlbm_batch_step(b, i % 2 ? f_b : f_a, i % 2 ? f_a : f_b, report ? 1 : 0, fields,
+                         report ? sums : nullptr, stream);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &graph);
+  if (rc != TLBM_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  tlbm_graph* g = new tlbm_graph();
+  g->graph = graph;
+  g->exec = exec;
+  *out = g;
+  return TLBM_OK;
+}
+
+int tlbm_graph_launch(tlbm_graph* g, void* stream) {
+  if (!g) return fail(TLBM_ERR_INVALID, "graph is NULL");
+  TLBM_HIP(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+  return TLBM_OK;
+}
+
+int tlbm_graph_destroy(tlbm_graph* g) {
+  if (!g) return TLBM_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return TLBM_OK;
+}
+
+int tlbm_device_name(char* buf, int len) {
+  if (!buf || len <= 0) return fail(TLBM_ERR_INVALID, "name buffer is NULL or empty");
+  int dev = 0;
+  TLBM_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  TLBM_HIP(hipGetDeviceProperties(&prop, dev));
+  snprintf(buf, static_cast<size_t>(len), "%s", prop.gcnArchName);
+  return TLBM_OK;
+}
+
+int tlbm_device_cus(int* cus) {
+  if (!cus) return fail(TLBM_ERR_INVALID, "cus is NULL");
+  int dev = 0;
+  TLBM_HIP(hipGetDevice(&dev));
+  TLBM_HIP(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
   return TLBM_OK;
 }
 

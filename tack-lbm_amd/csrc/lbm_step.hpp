@@ -382,29 +382,23 @@ __device__ __forceinline__ void tile_of_block(uint32_t b, uint32_t xblocks, uint
   }
 }
 
-// ---- the fused step --------------------------------------------------------------------------
+// ---- the fused step of one cell ------------------------------------------------------------------
+// Pull, bounce-back, collision, sponge and store for cell (x, y, z) of the lattice described by
+// `a`, reading `fin` and writing `fout` (plane stride a.ns).  In the single-lattice kernel y and z
+// are block-uniform (scalar registers, uniform branches); in the batched kernel they are per
+// lane.  REPORT: writes the 13 fields at `fidx` (when F.ux is not null) and returns the 14 FP64
+// summands in acc (zero for inactive lanes).
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE>
-__global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
+__device__ __forceinline__ void step_cell(const StepArgs& a, const typename StorageT<STORAGE>::T* __restrict__ fin,
+                                          typename StorageT<STORAGE>::T* __restrict__ fout,
+                                          int x, int y, int z, bool active,
+                                          const tlbm_fields& F, uint32_t fidx,
+                                          double (&acc)[TLBM_NSUMS]) {
   using Obs = ObstacleT<OBS>;
   using St = StorageT<STORAGE>;
   using T = typename St::T;
   constexpr bool kShift = St::kShift;
-  const T* __restrict__ fin = static_cast<const T*>(a.fin);
-  T* __restrict__ fout = static_cast<T*>(a.fout);
-
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  uint32_t xb, row;
-  tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
-  const int xr = static_cast<int>(xb * blockDim.x + threadIdx.x);
-  const bool active = xr < nx;
-  if constexpr (!REPORT) {
-    if (!active) return;
-  }
-  // in the report variant every lane reaches the block reduction: lanes past the end of the
-  // row mirror the last cell (in-bounds loads) and store nothing
-  const int x = active ? xr : nx - 1;
-  const int y = static_cast<int>(row % static_cast<uint32_t>(ny));
-  const int z = static_cast<int>(row / static_cast<uint32_t>(ny));
   const uint32_t snx = static_cast<uint32_t>(nx);
   const uint32_t plane = snx * static_cast<uint32_t>(ny);
   const uint32_t idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
@@ -511,43 +505,137 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
     });
   }
 
-  // ---- report fields and FP64 sums (lbm.py:493-513, 649-661) ----
+  // ---- report fields and FP64 summands (lbm.py:493-513, 649-661) ----
   if constexpr (REPORT) {
-    const tlbm_fields& F = a.fld;
-    double acc[TLBM_NSUMS];
 #pragma unroll
     for (int i = 0; i < TLBM_NSUMS; ++i) acc[i] = 0.0;
     if (active) {
       const float p = (rho - 1.0f) * (1.0f / 3.0f);
       const float usq = ux * ux + uy * uy + uz * uz;
       const float ke = 0.5f * rho * usq;
-      const float ox = F.ux[idx], oy = F.uy[idx], oz = F.uz[idx], op = F.p[idx];
-      const float ok = 0.5f * (ox * ox + oy * oy + oz * oz);
-      const float r_rho = (3.0f * (p - op)) * (3.0f * (p - op));
-      const float r_ux = (ux - ox) * (ux - ox);
-      const float r_uy = (uy - oy) * (uy - oy);
-      const float r_uz = (uz - oz) * (uz - oz);
-      const float r_ke = (ke - ok) * (ke - ok);
       const float vmag = fsqrt(usq);
-      F.r_rho[idx] = r_rho;
-      F.r_ux[idx] = r_ux;
-      F.r_uy[idx] = r_uy;
-      F.r_uz[idx] = r_uz;
-      F.r_ke[idx] = r_ke;
-      F.ux[idx] = ux;
-      F.uy[idx] = uy;
-      F.uz[idx] = uz;
-      F.p[idx] = p;
-      F.vmag[idx] = vmag;
-      F.fx[idx] = mx;
-      F.fy[idx] = my;
-      F.fz[idx] = mz;
+      float r_rho = 0.0f, r_ux = 0.0f, r_uy = 0.0f, r_uz = 0.0f, r_ke = 0.0f;
+      if (F.ux) {
+        const float ox = F.ux[fidx], oy = F.uy[fidx], oz = F.uz[fidx], op = F.p[fidx];
+        const float ok = 0.5f * (ox * ox + oy * oy + oz * oz);
+        r_rho = (3.0f * (p - op)) * (3.0f * (p - op));
+        r_ux = (ux - ox) * (ux - ox);
+        r_uy = (uy - oy) * (uy - oy);
+        r_uz = (uz - oz) * (uz - oz);
+        r_ke = (ke - ok) * (ke - ok);
+        F.r_rho[fidx] = r_rho;
+        F.r_ux[fidx] = r_ux;
+        F.r_uy[fidx] = r_uy;
+        F.r_uz[fidx] = r_uz;
+        F.r_ke[fidx] = r_ke;
+        F.ux[fidx] = ux;
+        F.uy[fidx] = uy;
+        F.uz[fidx] = uz;
+        F.p[fidx] = p;
+        F.vmag[fidx] = vmag;
+        F.fx[fidx] = mx;
+        F.fy[fidx] = my;
+        F.fz[fidx] = mz;
+      }
       const float vals[TLBM_NSUMS] = {ux,   uy,   uz,   p,  vmag, r_rho, r_ux,
                                       r_uy, r_uz, r_ke, mx, my,   mz,    rho};
 #pragma unroll
       for (int i = 0; i < TLBM_NSUMS; ++i) acc[i] = static_cast<double>(vals[i]);
     }
+  }
+}
+
+// ---- the step kernel of one lattice -------------------------------------------------------------
+template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE>
+__global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
+  using T = typename StorageT<STORAGE>::T;
+  uint32_t xb, row;
+  tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
+  const int xr = static_cast<int>(xb * blockDim.x + threadIdx.x);
+  const bool active = xr < a.nx;
+  if constexpr (!REPORT) {
+    if (!active) return;
+  }
+  // in the report variant every lane reaches the block reduction: lanes past the end of the
+  // row mirror the last cell (in-bounds loads) and store nothing
+  const int x = active ? xr : a.nx - 1;
+  const int y = static_cast<int>(row % static_cast<uint32_t>(a.ny));
+  const int z = static_cast<int>(row / static_cast<uint32_t>(a.ny));
+  const uint32_t idx = (static_cast<uint32_t>(z) * a.ny + y) * static_cast<uint32_t>(a.nx) + x;
+  double acc[TLBM_NSUMS];
+  step_cell<COLL, OBS, SIDE, REPORT, STORAGE>(a, static_cast<const T*>(a.fin),
+                                              static_cast<T*>(a.fout), x, y, z, active, a.fld,
+                                              idx, acc);
+  if constexpr (REPORT) {
     if (a.partials) block_sums(acc, a.partials, a.nblocks, blockIdx.x);
+  }
+}
+
+// ---- batched lattices: many independent domains in one persistent launch -------------------------
+// Domain d holds its populations at elem_off[d] (plane stride = its cell count) and its report
+// fields at cell_off[d] of the packed per-cell arrays.  The cells of all domains are cut into
+// chunks of blockDim.x consecutive cells that never straddle two domains; the persistent grid
+// walks the chunks with a block stride, so each block handles chunk c, c + gridDim.x, ...
+struct BatchDomain {
+  StepArgs a;          // the domain's parameters (pointers in it are unused)
+  uint64_t elem_off;   // first population element of the domain
+  uint32_t cell_off;   // first cell of the domain in packed per-cell arrays
+  uint32_t chunk_off;  // first chunk of the domain
+  float inv_nx, inv_nxny;  // for the idx -> (x, y, z) split
+  uint32_t pad[2];
+};
+
+struct BatchArgs {
+  const void* fin;
+  void* fout;
+  const BatchDomain* dom;
+  const uint16_t* chunk_dom;  // domain of each chunk
+  int32_t ndom;
+  uint32_t nchunks;
+  tlbm_fields fld;   // packed over all domains (cell_off), or all null
+  double* partials;  // report: [TLBM_NSUMS][nchunks]
+};
+
+// q = i / d, r = i % d for i < 2^24, with a float reciprocal and one correction each way.
+__device__ __forceinline__ uint32_t div_small(uint32_t i, uint32_t d, float inv, uint32_t& r) {
+  uint32_t q = static_cast<uint32_t>(static_cast<float>(i) * inv);
+  int32_t rr = static_cast<int32_t>(i) - static_cast<int32_t>(q * d);
+  if (rr < 0) {
+    q -= 1;
+    rr += static_cast<int32_t>(d);
+  }
+  if (rr >= static_cast<int32_t>(d)) {
+    q += 1;
+    rr -= static_cast<int32_t>(d);
+  }
+  r = static_cast<uint32_t>(rr);
+  return q;
+}
+
+template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE>
+__global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
+  using T = typename StorageT<STORAGE>::T;
+  const T* __restrict__ fin = static_cast<const T*>(b.fin);
+  T* __restrict__ fout = static_cast<T*>(b.fout);
+  for (uint32_t c = blockIdx.x; c < b.nchunks; c += gridDim.x) {
+    const BatchDomain& D = b.dom[b.chunk_dom[c]];
+    const StepArgs& a = D.a;
+    const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * a.nz;
+    const uint32_t local = (c - D.chunk_off) * blockDim.x + threadIdx.x;
+    const bool active = local < n;
+    const uint32_t cell = active ? local : n - 1;
+    uint32_t x, yz, y;
+    const uint32_t z = div_small(cell, static_cast<uint32_t>(a.nx) * a.ny, D.inv_nxny, yz);
+    y = div_small(yz, static_cast<uint32_t>(a.nx), D.inv_nx, x);
+    double acc[TLBM_NSUMS];
+    step_cell<COLL, OBS, SIDE, REPORT, STORAGE>(a, fin + D.elem_off, fout + D.elem_off,
+                                                static_cast<int>(x), static_cast<int>(y),
+                                                static_cast<int>(z), active, b.fld,
+                                                D.cell_off + cell, acc);
+    if constexpr (REPORT) {
+      if (b.partials) block_sums(acc, b.partials, b.nchunks, c);
+      __syncthreads();  // the block reuses its reduction scratch for the next chunk
+    }
   }
 }
 

@@ -17,13 +17,24 @@ hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t str
   return hipGetLastError();
 }
 
+using BatchLaunch = hipError_t (*)(const BatchArgs&, dim3 grid, dim3 block, hipStream_t stream);
+using BatchTable = BatchLaunch[3][3][2];
+
+template <int C, int O, int S, bool R, int ST>
+hipError_t launch_batch(const BatchArgs& b, dim3 grid, dim3 block, hipStream_t stream) {
+  hipLaunchKernelGGL((lbm_batch_kernel<C, O, S, R, ST>), grid, block, 0, stream, b);
+  return hipGetLastError();
+}
+
 template <int C, int ST>
-void fill_table(StepTable& t) {
+void fill_table(StepTable& t, BatchTable& bt) {
   static_for<0, 3>([&](auto o) {
     static_for<0, 3>([&](auto s) {
       constexpr int O = decltype(o)::value, S = decltype(s)::value;
       t[O][S][0] = &launch_step<C, O, S, false, ST>;
       t[O][S][1] = &launch_step<C, O, S, true, ST>;
+      bt[O][S][0] = &launch_batch<C, O, S, false, ST>;
+      bt[O][S][1] = &launch_batch<C, O, S, true, ST>;
     });
   });
 }
@@ -32,7 +43,8 @@ void fill_table(StepTable& t) {
 
 // One per (storage, collision) unit; defined in step_inst.hip compiled with
 // -DTLBM_STORAGE=<s> -DTLBM_COLL=<c>.
-#define TLBM_DECLARE_UNIT(s, c) void tlbm_register_##s##_##c(tlbm::StepTable& t);
+#define TLBM_DECLARE_UNIT(s, c) \
+  void tlbm_register_##s##_##c(tlbm::StepTable& t, tlbm::BatchTable& bt);
 TLBM_DECLARE_UNIT(0, 0)
 TLBM_DECLARE_UNIT(0, 1)
 TLBM_DECLARE_UNIT(0, 2)

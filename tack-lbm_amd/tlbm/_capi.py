@@ -73,6 +73,20 @@ SIGNATURES = {
     "tlbm_step": (_c.c_int, [_c.c_int, _vp, _vp, _c.POINTER(Params), _c.POINTER(Fields),
                              _vp, _vp]),
     "tlbm_copy": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
+    "tlbm_batch_create": (_c.c_int, [_c.c_int, _c.POINTER(Params), _c.c_int, _c.POINTER(_vp)]),
+    "tlbm_batch_destroy": (_c.c_int, [_vp]),
+    "tlbm_batch_layout": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_uint64),
+                                     _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64),
+                                     _c.POINTER(_c.c_uint64)]),
+    "tlbm_batch_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
+    "tlbm_batch_init": (_c.c_int, [_vp, _vp, _c.c_float, _c.c_float, _c.c_uint64, _vp]),
+    "tlbm_batch_step": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _c.POINTER(Fields), _vp, _vp]),
+    "tlbm_batch_graph": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _c.c_int, _c.POINTER(Fields), _vp,
+                                    _vp, _c.POINTER(_vp)]),
+    "tlbm_graph_launch": (_c.c_int, [_vp, _vp]),
+    "tlbm_graph_destroy": (_c.c_int, [_vp]),
+    "tlbm_device_cus": (_c.c_int, [_c.POINTER(_c.c_int)]),
+    "tlbm_device_name": (_c.c_int, [_c.c_char_p, _c.c_int]),
     "tlbm_coarsen": (_c.c_int, [_vp, _vp, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int,
                                 _c.c_int, _c.c_int, _vp]),
 }

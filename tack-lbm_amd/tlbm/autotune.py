@@ -1,0 +1,133 @@
+"""Launch-shape autotuner for the step kernels.
+
+Times each candidate launch shape on the device with HIP events and keeps the fastest, caching
+the choice in a JSON file keyed by device and workload (``$TLBM_AUTOTUNE_CACHE``, default
+``~/.cache/tlbm/autotune.json``).  Tuning steps from the current state into the second
+population buffer without swapping, so it does not advance or alter the simulation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+from . import _capi
+from ._capi import check, lib
+
+CACHE = os.environ.get("TLBM_AUTOTUNE_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "tlbm", "autotune.json"))
+
+
+def _load():
+    try:
+        with open(CACHE) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        tmp = CACHE + ".tmp"
+        with open(tmp, "w") as fh:
+            json.dump(table, fh, indent=1, sort_keys=True)
+        os.replace(tmp, CACHE)
+    except OSError:
+        pass  # a read-only home: tuning still applies to this process
+
+
+def device_key():
+    buf = ctypes.create_string_buffer(256)
+    check(lib().tlbm_device_name(buf, 256), "device name")
+    cus = ctypes.c_int()
+    check(lib().tlbm_device_cus(ctypes.byref(cus)), "device CUs")
+    return f"{buf.value.decode(errors='replace')}/{cus.value}cu"
+
+
+def _time(launch, stream, reps):
+    L = lib()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    check(L.tlbm_event_create(ctypes.byref(a)), "event")
+    check(L.tlbm_event_create(ctypes.byref(b)), "event")
+    try:
+        launch()
+        launch()
+        check(L.tlbm_event_record(a, stream), "record")
+        for _ in range(reps):
+            launch()
+        check(L.tlbm_event_record(b, stream), "record")
+        ms = ctypes.c_float()
+        check(L.tlbm_event_elapsed_ms(a, b, ctypes.byref(ms)), "elapsed")
+        return ms.value / reps
+    finally:
+        L.tlbm_event_destroy(a)
+        L.tlbm_event_destroy(b)
+
+
+def tune_batch(batch, reps=5, use_cache=True, candidates=None):
+    """Pick (threads per block, persistent grid) for a BatchSolver; returns (block, grid, ms)."""
+    L = lib()
+    cus = ctypes.c_int()
+    check(L.tlbm_device_cus(ctypes.byref(cus)), "device CUs")
+    key = (f"batch|{device_key()}|h{batch.handle}|{len(batch.domains)}dom|{batch.cells}cells|"
+           + ",".join(f"{d.nx}x{d.ny}x{d.nz}" for d in batch.domains[:4]))
+    table = _load() if use_cache else {}
+    if key in table:
+        block, grid, ms = table[key]
+        batch.set_launch(block, grid)
+        return block, grid, ms
+    if candidates is None:
+        candidates = []
+        for block in (64, 128, 256):
+            waves = block // 64
+            for per_simd in (0, 2, 4, 8):  # 0: one block per chunk (not persistent)
+                grid = 0 if per_simd == 0 else cus.value * 4 * per_simd // waves
+                candidates.append((block, grid))
+    best = None
+    for block, grid in candidates:
+        batch.set_launch(block, grid)
+
+        def launch():
+            check(L.tlbm_batch_step(batch._batch, batch.f[0].ptr, batch.f[1].ptr, 0, None, None,
+                                    batch.stream), "batch step")
+        ms = _time(launch, batch.stream, reps)
+        if best is None or ms < best[2]:
+            best = (block, grid, ms)
+    batch.set_launch(best[0], best[1])
+    if use_cache:
+        table[key] = list(best)
+        _store(table)
+    return best
+
+
+def tune_solver(solver, reps=5, use_cache=True, blocks=(64, 128, 256)):
+    """Pick threads per block for a Solver's quiet kernel (the report kernel follows it)."""
+    L = lib()
+    lat = solver.lat
+    key = (f"solver|{device_key()}|{solver.collision_name}|{type(solver.geo).__name__}|"
+           f"{type(solver.flow.side).__name__}|{solver.storage}|{lat.nx}x{lat.ny}x{lat.nz}")
+    table = _load() if use_cache else {}
+    if key in table:
+        block, ms = table[key]
+        solver.set_launch(block)
+        return block, ms
+    from .lbm import make_params
+    params = make_params(lat, solver.phys, solver.geo, solver.flow, solver.qstride)
+    best = None
+    for block in blocks:
+        solver.set_launch(block)
+        h = solver._handle(report=False)
+
+        def launch():
+            check(L.tlbm_step(h, solver.f[0].ptr, solver.f[1].ptr, ctypes.byref(params), None,
+                              None, solver.stream), "step")
+        ms = _time(launch, solver.stream, reps)
+        if best is None or ms < best[1]:
+            best = (block, ms)
+    solver.set_launch(best[0])
+    if use_cache:
+        table[key] = list(best)
+        _store(table)
+    return best

@@ -591,6 +591,12 @@ class Solver:
             raise KeyError(f"obstacle {kind!r} is not built (have {sorted(makers)})")
         self.geo = makers[kind]()
 
+    def _handle(self, report=False):
+        """The kernel handle of this solver's quiet or report step."""
+        return kernel_handle(_code(self.phys, "collision"), _code(self.geo, "obstacle"),
+                             _capi.REPORT if report else _capi.QUIET,
+                             _code(self.flow.side, "side"), STORAGES[self.storage])
+
     def step(self, report=False):
         """lbm.py:639-647: one step, then swap the ping-pong buffers."""
         diag = self.report if report else self.quiet

@@ -1,0 +1,153 @@
+"""Batched lattices (parameter sweeps): one launch steps many domains of different sizes.
+
+Each domain of a batch must step exactly as the single-lattice kernel steps it (the same
+per-cell arithmetic, so bit for bit) and as the FP64 oracle steps it (to FP32 rounding); the
+per-domain report sums must equal the single-lattice ones; every launch shape and the hipGraph
+replay must give the same state as eager steps.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import lbm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _domains():
+    from tlbm.batch import Domain
+    # uneven sizes (rows shorter and longer than a block, odd extents), Re and radii
+    return [Domain(20, 10, 10, 150.0, 5.0), Domain(64, 32, 32, 400.0, 9.0),
+            Domain(37, 12, 8, 900.0, 6.5), Domain(130, 16, 12, 250.0, 8.0),
+            Domain(8, 6, 4, 100.0, 3.0)]
+
+
+def _random_state(batch, seed):
+    import tlbm
+    rng = np.random.default_rng(seed)
+    raw = np.empty(batch.elements, batch.f[0].dtype)
+    for d, dom in enumerate(batch.domains):
+        e, _ = batch.offsets(d)
+        rho = 1 + 0.01 * rng.standard_normal(dom.n)
+        u = 0.05 * rng.standard_normal((3, dom.n))
+        f = np.stack([O.feq(q, rho, *u) for q in range(19)]) + 1e-3 * rng.standard_normal(
+            (19, dom.n))
+        raw[e:e + 19 * dom.n] = tlbm.encode_populations(f.astype(np.float32),
+                                                        batch.storage).reshape(-1)
+    batch.f[0].copy_from(raw)
+    return raw
+
+
+def _single_step(batch, d, raw, report):
+    """Domain d stepped alone by the single-lattice kernel from the same populations."""
+    import tlbm
+    dom = batch.domains[d]
+    lat = tlbm.D3Q19(dom.nx, dom.ny, dom.nz)
+    e, _ = batch.offsets(d)
+    a = tlbm.array(raw[e:e + 19 * dom.n], batch.storage)
+    out = tlbm.empty((19 * dom.n,), batch.storage)
+    fields = {k: tlbm.zeros((dom.n,)) for k in tlbm.FIELDS}
+    sums = tlbm.HostBuffer(14)
+    flow = tlbm.Flow(batch.u0, 0.75 * dom.nx, 1.0, batch._omega, tlbm.PeriodicY())
+    diag = tlbm.Report(fields) if report else tlbm.Quiet(fields)
+    tlbm.lbm_step(lat, batch.phys[d], batch.geo[d], flow, diag, a, out, sums=sums)
+    tlbm.sync()
+    return out.numpy(), sums.numpy.copy()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
+def test_batch_step_matches_single_lattice_and_oracle(storage, collision):
+    import tlbm
+    from tlbm.batch import BatchSolver
+    b = BatchSolver(_domains(), collision=collision, storage=storage, cs=0.1, magic=0.25)
+    raw = _random_state(b, seed=5)
+    b.step(report=True)
+    tlbm.sync()
+    got = b.f[0].numpy()
+    sums = b.sums.numpy.reshape(len(b.domains), 14)
+    for d, dom in enumerate(b.domains):
+        e, _ = b.offsets(d)
+        single, ssums = _single_step(b, d, raw, report=True)
+        assert np.array_equal(got[e:e + 19 * dom.n], single), f"domain {d}"
+        # report sums: the single kernel used old fields = 0; the batch ran without fields, so
+        # compare the sums that do not involve residuals (u, p, |u|, forces, mass)
+        for k in (0, 1, 2, 3, 4, 10, 11, 12, 13):
+            assert abs(sums[d, k] - ssums[k]) <= 1e-9 * max(1.0, abs(ssums[k])), (d, k)
+        if storage == "f32":
+            g = b.geo[d]
+            geo = O.Cylinder(g.x, g.y, g.r)
+            phys = {"bgk": O.BGK, "trt": lambda nu: O.TRT(nu, 0.25, 0.1),
+                    "les": lambda nu: O.LES(nu, 0.1)}[collision](b.u0 * dom.diameter / dom.reynolds)
+            ref = O.step((dom.nx, dom.ny, dom.nz), phys, geo, O.Flow(b.u0, 0.75 * dom.nx, 1.0),
+                         raw[e:e + 19 * dom.n].astype(np.float64).reshape(19, dom.n))
+            err = np.abs(got[e:e + 19 * dom.n].reshape(19, -1) - ref).max() / np.abs(ref).max()
+            assert err < 1e-6, (d, err)
+
+
+@pytest.mark.parametrize("block,grid", [(64, 0), (128, 0), (256, 0), (64, 7), (256, 3),
+                                        (128, 1000)])
+def test_batch_launch_shapes_agree(block, grid):
+    import tlbm
+    from tlbm.batch import BatchSolver
+    b = BatchSolver(_domains(), collision="trt", storage="f32")
+    _random_state(b, seed=6)
+    b.step()
+    ref = b.f[0].numpy()
+    c = BatchSolver(_domains(), collision="trt", storage="f32")
+    c.set_launch(block, grid)
+    _random_state(c, seed=6)
+    c.step()
+    tlbm.sync()
+    assert np.array_equal(c.f[0].numpy(), ref)
+
+
+def test_batch_graph_replays_match_eager_steps():
+    """Each replay of the captured graph (with eager work and syncs between replays) leaves the
+    state that the same number of eager steps leaves."""
+    import tlbm
+    from tlbm.batch import BatchSolver
+    L = tlbm._capi.lib()
+    stream = ctypes.c_void_p()
+    tlbm._capi.check(L.tlbm_stream_create(ctypes.byref(stream)), "stream")
+    g = BatchSolver(_domains(), collision="les", storage="f16", stream=stream.value,
+                    noise_amp=1e-3)
+    e = BatchSolver(_domains(), collision="les", storage="f16", noise_amp=1e-3)
+    for replay in range(3):
+        g.run(4, report_every=2)
+        for i in range(4):
+            e.step(report=(i % 2 == 1))
+        tlbm.sync(stream.value)
+        tlbm.sync()
+        assert np.array_equal(g.f[0].numpy(), e.f[0].numpy()), replay
+        assert np.allclose(g.sums.numpy, e.sums.numpy, rtol=1e-12, atol=1e-12), replay
+        e.f[1].zero_()  # eager work between replays
+    g.close()
+    L.tlbm_stream_destroy(stream)
+
+
+def test_batch_autotune_keeps_state(tmp_path, monkeypatch):
+    import tlbm
+    from tlbm import autotune
+    from tlbm.batch import BatchSolver
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "tune.json"))
+    b = BatchSolver(_domains(), collision="trt", storage="f32")
+    _random_state(b, seed=7)
+    before = b.f[0].numpy()
+    block, grid, ms = autotune.tune_batch(b, reps=2)
+    assert block in (64, 128, 256) and ms > 0
+    assert np.array_equal(b.f[0].numpy(), before)
+    again = autotune.tune_batch(b, reps=2)
+    assert again[:2] == (block, grid)  # from the cache
+
+
+def test_sweep_domains_are_seeded_and_in_range():
+    from tlbm.batch import sweep_domains
+    a = sweep_domains(64, seed=3)
+    b = sweep_domains(64, seed=3)
+    assert [(d.nx, d.reynolds, d.diameter) for d in a] == [(d.nx, d.reynolds, d.diameter) for d in b]
+    for d in a:
+        assert 64 <= d.nx <= 192 and d.ny == d.nz == d.nx // 2
+        assert 100 <= d.reynolds <= 1000 and 8 <= d.diameter <= 24
