@@ -13,7 +13,11 @@ written), over its mean duration from HIP events on the launch stream.
 cpu_baseline: the NumPy FP32 restatement of the same step (the oracle in float32), on a 512x256x4
 slab of the same workload, single-threaded.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--storage f32,f16,bf16]
+sweep: the batched parameter sweep (64 uneven TRT channels, one persistent launch per step from
+a hipGraph), FP32 and FP16 storage, reported under "sweep".
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
+                       [--storage f32,f16,bf16] [--block N] [--calibrate]
 """
 
 import argparse
@@ -166,6 +170,60 @@ def calibrate_copy(gib=1, reps=5):
             "gbs": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
+def time_sweep(storage, steps=5000, graph_steps=100, seq_steps=20):
+    """The parameter sweep: 64 seeded TRT + Smagorinsky channels (nx in [64, 192], ny = nz =
+    nx / 2, Re in [100, 1000], cylinder radius in [4, 12], u0 = 0.08, seeded 1e-3 noise) stepped
+    `steps` times by one persistent launch per step, replayed from a hipGraph of `graph_steps`
+    steps with a report step at its end; launch shape from the autotuner.  For comparison, the
+    same domains stepped one launch per domain (one Solver each) for `seq_steps` steps."""
+    import tlbm
+    from tlbm import autotune
+    from tlbm.batch import BatchSolver, sweep_domains
+    from tlbm._capi import check, lib
+    L = lib()
+    domains = sweep_domains(64, seed=0)
+    stream = ctypes.c_void_p()
+    check(L.tlbm_stream_create(ctypes.byref(stream)), "stream create")
+    b = BatchSolver(domains, collision="trt", storage=storage, noise_amp=1e-3,
+                    stream=stream.value)
+    block, grid, tune_ms = autotune.tune_batch(b)
+    b.run(graph_steps, report_every=graph_steps)  # capture + warm-up
+    ev = Events(L)
+    e0, e1 = ev.new(), ev.new()
+    tlbm.sync(stream.value)
+    check(L.tlbm_event_record(e0, stream), "record")
+    for _ in range(steps // graph_steps):
+        b.run(graph_steps, report_every=graph_steps)
+    check(L.tlbm_event_record(e1, stream), "record")
+    ms = ev.elapsed(e0, e1)
+    mons = b.monitors(graph_steps)
+    cells = b.cells
+    b.close()
+    del b
+    # one launch per domain per step
+    seq = [tlbm.Solver(d.nx, d.ny, d.nz, collision="trt", obstacle="cylinder",
+                       reynolds=d.reynolds, u0=0.08, diameter=d.diameter, storage=storage,
+                       stream=stream.value) for d in domains]
+    for s in seq:
+        s.step()
+    tlbm.sync(stream.value)
+    check(L.tlbm_event_record(e0, stream), "record")
+    for _ in range(seq_steps):
+        for s in seq:
+            s.step()
+    check(L.tlbm_event_record(e1, stream), "record")
+    seq_ms = ev.elapsed(e0, e1)
+    del seq
+    L.tlbm_stream_destroy(stream)
+    bpc = BYTES_PER_CELL[storage]
+    return {"domains": len(domains), "cells": cells, "steps": steps,
+            "mlups": cells * steps / (ms * 1e-3) / 1e6, "ms_per_step": ms / steps,
+            "achieved_gbs": bpc * cells * steps / (ms * 1e-3) / 1e9,
+            "launch": {"block": block, "persistent_grid": grid, "tuned_ms": round(tune_ms, 4)},
+            "one_launch_per_domain_mlups": cells * seq_steps / (seq_ms * 1e-3) / 1e6,
+            "drag_range": [min(m["drag"] for m in mons), max(m["drag"] for m in mons)]}
+
+
 def cpu_baseline(max_seconds=20.0):
     """The NumPy float32 restatement on a 512x256x4 slab of the workload, one core."""
     import numpy as np
@@ -197,6 +255,7 @@ def main():
     ap.add_argument("--storage", default="f32,f16,bf16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0 = default)")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched parameter sweep")
     ap.add_argument("--calibrate", action="store_true",
                     help="also time a plain 1 GiB copy (counter calibration, streaming ceiling)")
     args = ap.parse_args()
@@ -211,6 +270,13 @@ def main():
         print(f"[bench] {st}: {results[st]['mlups']:.0f} MLUPS, quiet kernel "
               f"{results[st]['quiet_kernel_ms']:.3f} ms = {results[st]['achieved_gbs']:.0f} GB/s",
               file=sys.stderr, flush=True)
+    sweep = {}
+    if not args.no_sweep:
+        for st in ("f32", "f16"):
+            sweep[st] = time_sweep(st)
+            print(f"[bench] sweep {st}: {sweep[st]['mlups']:.0f} MLUPS over {sweep[st]['cells']} "
+                  f"cells in {sweep[st]['domains']} domains (one launch per domain: "
+                  f"{sweep[st]['one_launch_per_domain_mlups']:.0f})", file=sys.stderr, flush=True)
     copy = calibrate_copy() if args.calibrate else None
     if copy:
         print(f"[bench] copy kernel: {copy['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -246,6 +312,9 @@ def main():
                           "drag": r["monitors"]["drag"], "lift": r["monitors"]["lift"]}
                      for st, r in results.items()},
     }
+    if sweep:
+        line["sweep"] = {st: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+                         for st, r in sweep.items()}
     if copy:
         line["copy_calibration"] = {k: round(v, 4) if isinstance(v, float) else v
                                     for k, v in copy.items()}

@@ -574,8 +574,8 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
 // ---- batched lattices: many independent domains in one persistent launch -------------------------
 // Domain d holds its populations at elem_off[d] (plane stride = its cell count) and its report
 // fields at cell_off[d] of the packed per-cell arrays.  The cells of all domains are cut into
-// chunks of blockDim.x consecutive cells that never straddle two domains; the persistent grid
-// walks the chunks with a block stride, so each block handles chunk c, c + gridDim.x, ...
+// chunks of blockDim.x consecutive cells that never straddle two domains; a (persistent) grid
+// walks the chunk slots with a block stride.
 struct BatchDomain {
   StepArgs a;          // the domain's parameters (pointers in it are unused)
   uint64_t elem_off;   // first population element of the domain
@@ -612,12 +612,22 @@ __device__ __forceinline__ uint32_t div_small(uint32_t i, uint32_t d, float inv,
   return q;
 }
 
+// Chunks are handed to XCDs in contiguous runs: XCD j (blocks j, j + 8, ...) takes chunks
+// [j * per, (j + 1) * per) in order, so consecutive chunks of a domain, which share a cache line
+// at their edge (the x +- 1 pulls), meet in the same L2.
+__device__ __forceinline__ uint32_t chunk_of_slot(uint32_t slot, uint32_t per) {
+  return (slot % kXcds) * per + slot / kXcds;
+}
+
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE>
 __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
   using T = typename StorageT<STORAGE>::T;
   const T* __restrict__ fin = static_cast<const T*>(b.fin);
   T* __restrict__ fout = static_cast<T*>(b.fout);
-  for (uint32_t c = blockIdx.x; c < b.nchunks; c += gridDim.x) {
+  const uint32_t per = (b.nchunks + kXcds - 1) / kXcds;
+  for (uint32_t slot = blockIdx.x; slot < per * kXcds; slot += gridDim.x) {
+    const uint32_t c = chunk_of_slot(slot, per);
+    if (c >= b.nchunks) continue;  // uniform: the whole block skips the empty slot
     const BatchDomain& D = b.dom[b.chunk_dom[c]];
     const StepArgs& a = D.a;
     const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * a.nz;
