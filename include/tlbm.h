@@ -32,8 +32,11 @@ extern "C" {
 
 /* collision models: lbm.py:544 COLLISIONS = {"bgk": BGK, "trt": TRT, "les": SmagorinskyLES} */
 enum { TLBM_BGK = 0, TLBM_TRT = 1, TLBM_LES = 2 };
-/* obstacles: lbm.py:144-227 (Cylinder, Sphere, Box) */
-enum { TLBM_CYLINDER = 0, TLBM_SPHERE = 1, TLBM_BOX = 2 };
+/* obstacles: lbm.py:144-227 (Cylinder, Sphere, Box); TLBM_FIELD has no counterpart in the
+ * reference: a per-cell solid fraction phi in [0, 1] (tlbm_params.phi), used as the `solid`
+ * value of lbm.py:403 and 416 -- partial bounce-back f = pulled + phi (bounced - pulled) on each
+ * link, the smoothed, differentiable obstacle of the adjoint. */
+enum { TLBM_CYLINDER = 0, TLBM_SPHERE = 1, TLBM_BOX = 2, TLBM_FIELD = 3 };
 /* diagnostics variants: lbm.py:325-339 (Quiet.enabled = 0, Report.enabled = 1) */
 enum { TLBM_QUIET = 0, TLBM_REPORT = 1 };
 /* lateral y sides: lbm.py:342-369 (PeriodicY, SlipY); TLBM_LID_Y has no counterpart in the
@@ -50,7 +53,7 @@ enum { TLBM_F32 = 0, TLBM_F16 = 1, TLBM_BF16 = 2 };
 
 typedef struct {
   int32_t collision; /* TLBM_BGK / TLBM_TRT / TLBM_LES           */
-  int32_t obstacle;  /* TLBM_CYLINDER / TLBM_SPHERE / TLBM_BOX    */
+  int32_t obstacle;  /* TLBM_CYLINDER / TLBM_SPHERE / TLBM_BOX / TLBM_FIELD */
   int32_t diag;      /* TLBM_QUIET / TLBM_REPORT                  */
   int32_t side;      /* TLBM_PERIODIC_Y / TLBM_SLIP_Y / TLBM_LID_Y */
   int32_t storage;   /* TLBM_F32 / TLBM_F16 / TLBM_BF16           */
@@ -74,6 +77,7 @@ typedef struct {
   const float* omega_rot;  /* device/managed pointer to one float, the cylinder surface
                               speed read by every step (Flow.omega_rot, lbm.py:378-379,
                               417); NULL means 0                                          */
+  const float* phi;        /* TLBM_FIELD: device pointer to nx*ny*nz solid fractions      */
 } tlbm_params;
 
 /* The 13 report fields of lbm.py:545, each n = nx*ny*nz floats (device or managed). */

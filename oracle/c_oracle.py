@@ -22,7 +22,8 @@ class Params(ctypes.Structure):
                 ("cs", ctypes.c_double), ("magic", ctypes.c_double),
                 ("obs", ctypes.c_double * 6), ("u0", ctypes.c_double),
                 ("sponge_start", ctypes.c_double), ("sponge_strength", ctypes.c_double),
-                ("lid", ctypes.c_double * 3), ("omega_rot", ctypes.c_double)]
+                ("lid", ctypes.c_double * 3), ("omega_rot", ctypes.c_double),
+                ("phi", ctypes.c_void_p)]
 
 
 _lib = None
@@ -49,7 +50,7 @@ def params(dims, phys, geo, flow):
     p = Params()
     p.nx, p.ny, p.nz = dims
     p.collision = {"bgk": 0, "trt": 1, "les": 2}[phys.kind] if phys is not None else 0
-    p.obstacle = {"cylinder": 0, "sphere": 1, "box": 2}[geo.kind]
+    p.obstacle = {"cylinder": 0, "sphere": 1, "box": 2, "field": 3}[geo.kind]
     p.side = {"periodic": 0, "slip": 1, "lid": 2}[flow.side.kind]
     if phys is not None:
         p.omega = getattr(phys, "omega", 0.0)
@@ -57,9 +58,11 @@ def params(dims, phys, geo, flow):
         p.cs = getattr(phys, "cs", 0.0)
         p.magic = getattr(phys, "magic", 0.0)
     vals = {"cylinder": ("x", "y", "r"), "sphere": ("x", "y", "z", "r"),
-            "box": ("x", "y", "z", "hx", "hy", "hz")}[geo.kind]
+            "box": ("x", "y", "z", "hx", "hy", "hz"), "field": ()}[geo.kind]
     for i, name in enumerate(vals):
         p.obs[i] = float(getattr(geo, name))
+    if geo.kind == "field":
+        p.phi = geo.phi.ctypes.data  # kept alive by geo
     p.u0 = flow.u0
     p.sponge_start = flow.sponge_start
     p.sponge_strength = flow.sponge_strength

@@ -35,6 +35,7 @@ typedef struct {
   double u0, sponge_start, sponge_strength;
   double lid[3];
   double omega_rot;
+  const float* phi; /* obstacle 3: per-cell solid fraction */
 } oracle_params;
 
 /* lbm.py:28-35 by search */
@@ -60,6 +61,7 @@ static double feq(int q, double rho, double ux, double uy, double uz) {
 /* lbm.py:150-153, 178-182, 208-211: float32, one rounding per operation */
 static double solid(const oracle_params* p, int x, int y, int z) {
   const float fx = (float)x, fy = (float)y, fz = (float)z;
+  if (p->obstacle == 3) return p->phi[((int64_t)z * p->ny + y) * p->nx + x];
   if (p->obstacle == 0) {
     const float dx = fx - (float)p->obs[0], dy = fy - (float)p->obs[1];
     const float r = (float)p->obs[2];
@@ -145,18 +147,18 @@ int oracle_step(const oracle_params* p, const double* fin, double* fout, double*
   const int nx = p->nx, ny = p->ny, nz = p->nz;
   const int64_t n = (int64_t)nx * ny * nz;
   /* solid() of every cell, evaluated once (the reference evaluates it per use) */
-  unsigned char* mask = (unsigned char*)malloc((size_t)n);
+  double* mask = (double*)malloc((size_t)n * sizeof(double));
   if (!mask) return 1;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int z = 0; z < nz; ++z)
     for (int y = 0; y < ny; ++y)
-      for (int x = 0; x < nx; ++x) mask[((int64_t)z * ny + y) * nx + x] = solid(p, x, y, z) > 0;
+      for (int x = 0; x < nx; ++x) mask[((int64_t)z * ny + y) * nx + x] = solid(p, x, y, z);
 #pragma omp parallel for collapse(2) schedule(static)
   for (int z = 0; z < nz; ++z)
     for (int y = 0; y < ny; ++y)
       for (int x = 0; x < nx; ++x) {
         const int64_t idx = ((int64_t)z * ny + y) * nx + x;
-        const double here = mask[idx] ? 1.0 : 0.0;
+        const double here = mask[idx];
         const double fluid = 1.0 - here;
         double f[19], post[19];
         double mx = 0, my = 0, mz = 0;
@@ -182,7 +184,7 @@ int oracle_step(const oracle_params* p, const double* fin, double* fout, double*
             f[q] = bounced + w6 * cdl;
             continue;
           }
-          const double wall = mask[((int64_t)zs * ny + ys) * nx + xs] ? 1.0 : 0.0;
+          const double wall = mask[((int64_t)zs * ny + ys) * nx + xs];
           double cdu = 0.0;
           if (wall > 0) {
             double uw[3];

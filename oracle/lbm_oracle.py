@@ -220,6 +220,24 @@ class Box:
         return zero, zero, zero
 
 
+class Field:
+    """A per-cell solid fraction phi (float32, flat over idx) used as ``solid`` (no counterpart
+    in the reference; lbm.py:403, 416, 428-429 and 449 read with a fractional value give partial
+    bounce-back).  Resting wall."""
+    kind = "field"
+
+    def __init__(self, phi, dims):
+        self.phi = np.asarray(phi, np.float32).reshape(-1)
+        self.nx, self.ny, self.nz = dims
+
+    def solid(self, x, y, z):
+        return self.phi[(np.asarray(z) * self.ny + np.asarray(y)) * self.nx + np.asarray(x)]
+
+    def wall_vel(self, x, y, z, dt):
+        zero = np.zeros_like(x)
+        return zero, zero, zero
+
+
 # --- lateral sides: lbm.py:342-369, plus the moving top lid --------------------------------------
 class PeriodicY:
     """lbm.py:342-352: y wraps."""

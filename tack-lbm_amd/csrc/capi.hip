@@ -43,7 +43,7 @@ int hip_fail(hipError_t e, const char* what) {
   } while (0)
 
 // ---- specialisation table ------------------------------------------------------------------
-constexpr int kHandles = 3 * 3 * 3 * 3 * 2;  // storage, collision, obstacle, side, diag
+constexpr int kHandles = 3 * 3 * 4 * 3 * 2;  // storage, collision, obstacle, side, diag
 
 tlbm::StepTable g_tables[3][3];
 tlbm::BatchTable g_batch_tables[3][3];
@@ -74,8 +74,8 @@ bool decode(int h, Decoded& d) {
   h /= 2;
   d.side = h % 3;
   h /= 3;
-  d.obstacle = h % 3;
-  h /= 3;
+  d.obstacle = h % 4;
+  h /= 4;
   d.collision = h % 3;
   h /= 3;
   d.storage = h;
@@ -145,8 +145,10 @@ __global__ void __launch_bounds__(256) copy16_kernel(const uint4* __restrict__ s
   if (i < n16) dst[i] = src[i];
 }
 
-int check_params(const tlbm_params* p) {
+int check_params(const tlbm_params* p, int obstacle = -1) {
   if (!p) return fail(TLBM_ERR_INVALID, "params is NULL");
+  if (obstacle == TLBM_FIELD && !p->phi)
+    return fail(TLBM_ERR_INVALID, "a TLBM_FIELD obstacle needs params->phi");
   if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0)
     return fail(TLBM_ERR_INVALID, "grid %dx%dx%d must be positive", p->nx, p->ny, p->nz);
   const uint64_t n = static_cast<uint64_t>(p->nx) * p->ny * p->nz;
@@ -180,17 +182,18 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
   } else if (obstacle == TLBM_SPHERE) {
     r = p.obs[3];
     a.reach[0] = a.reach[1] = a.reach[2] = std::fabs(r) + 2.0f;
-  } else {
+  } else if (obstacle == TLBM_BOX) {
     a.reach[0] = std::fabs(p.obs[3]) + 2.0f;
     a.reach[1] = std::fabs(p.obs[4]) + 2.0f;
     a.reach[2] = std::fabs(p.obs[5]) + 2.0f;
   }
+  a.phi = p.phi;
   a.r2 = r * r;
   // periodic faces the obstacle reaches: cells on those faces test their wrapped sources
   const float ctr[3] = {p.obs[0], p.obs[1], obstacle == TLBM_CYLINDER ? 0.0f : p.obs[2]};
   const int ext[3] = {p.nx, p.ny, p.nz};
-  a.wrap = 0;
-  for (int d = 0; d < 3; ++d) {
+  a.wrap = obstacle == TLBM_FIELD ? 7 : 0;  // a field may be solid anywhere
+  for (int d = 0; d < 3 && obstacle != TLBM_FIELD; ++d) {
     if (d == 2 && obstacle == TLBM_CYLINDER) continue;  // infinite along z: z wraps change nothing
     const float lo = std::fabs(0.0f - ctr[d]), hi = std::fabs(static_cast<float>(ext[d] - 1) - ctr[d]);
     if (lo < a.reach[d] || hi < a.reach[d] || !std::isfinite(lo) || !std::isfinite(hi))
@@ -222,10 +225,11 @@ hipError_t launch_init(const tlbm::StepArgs& a, float seed_amp, float noise_amp,
 
 using InitLaunch = hipError_t (*)(const tlbm::StepArgs&, float, float, uint64_t, dim3, dim3,
                                   hipStream_t);
-const InitLaunch kInit[3][3] = {
-    {&launch_init<0, 0>, &launch_init<1, 0>, &launch_init<2, 0>},
-    {&launch_init<0, 1>, &launch_init<1, 1>, &launch_init<2, 1>},
-    {&launch_init<0, 2>, &launch_init<1, 2>, &launch_init<2, 2>}};  // [storage][obstacle]
+const InitLaunch kInit[3][4] = {
+    {&launch_init<0, 0>, &launch_init<1, 0>, &launch_init<2, 0>, &launch_init<3, 0>},
+    {&launch_init<0, 1>, &launch_init<1, 1>, &launch_init<2, 1>, &launch_init<3, 1>},
+    {&launch_init<0, 2>, &launch_init<1, 2>, &launch_init<2, 2>, &launch_init<3, 2>}};
+// [storage][obstacle]
 
 // Per-domain totals of a batched report step: block (d, k) sums partials[k][chunk] over the
 // chunks of domain d in a fixed order.
@@ -483,11 +487,11 @@ int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle) {
   std::call_once(g_once, build_tables);
   const tlbm_spec& s = *spec;
   if (s.collision < 0 || s.collision > 2) return fail(TLBM_ERR_INVALID, "collision %d", s.collision);
-  if (s.obstacle < 0 || s.obstacle > 2) return fail(TLBM_ERR_INVALID, "obstacle %d", s.obstacle);
+  if (s.obstacle < 0 || s.obstacle > 3) return fail(TLBM_ERR_INVALID, "obstacle %d", s.obstacle);
   if (s.diag < 0 || s.diag > 1) return fail(TLBM_ERR_INVALID, "diag %d", s.diag);
   if (s.side < 0 || s.side > 2) return fail(TLBM_ERR_INVALID, "side %d", s.side);
   if (s.storage < 0 || s.storage > 2) return fail(TLBM_ERR_INVALID, "storage %d", s.storage);
-  *handle = ((((s.storage * 3 + s.collision) * 3 + s.obstacle) * 3 + s.side) * 2) + s.diag;
+  *handle = ((((s.storage * 3 + s.collision) * 4 + s.obstacle) * 3 + s.side) * 2) + s.diag;
   return TLBM_OK;
 }
 
@@ -507,7 +511,7 @@ int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, fl
   Decoded d;
   if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
   if (!f) return fail(TLBM_ERR_INVALID, "f is NULL");
-  if (int rc = check_params(params)) return rc;
+  if (int rc = check_params(params, d.obstacle)) return rc;
   tlbm::StepArgs a = make_args(*params, d.obstacle);
   a.fout = f;
   const uint64_t xblocks = (params->nx + 255) / 256;
@@ -531,7 +535,7 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
   if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
   if (!f_in || !f_out) return fail(TLBM_ERR_INVALID, "f_in or f_out is NULL");
   if (f_in == f_out) return fail(TLBM_ERR_INVALID, "f_in and f_out must be distinct buffers");
-  if (int rc = check_params(params)) return rc;
+  if (int rc = check_params(params, d.obstacle)) return rc;
   tlbm::StepArgs a = make_args(*params, d.obstacle);
   a.fin = f_in;
   a.fout = f_out;
@@ -617,7 +621,7 @@ int tlbm_batch_create(int handle, const tlbm_params* params, int ndomains, tlbm_
   for (int i = 0; i < ndomains; ++i) {
     tlbm_params p = params[i];
     p.qstride = static_cast<int64_t>(p.nx) * p.ny * p.nz;
-    if (int rc = check_params(&p)) return rc;
+    if (int rc = check_params(&p, d.obstacle)) return rc;
     if (p.qstride >= (1 << 24))
       return fail(TLBM_ERR_INVALID, "domain %d has %lld cells; batched domains hold < 2^24", i,
                   static_cast<long long>(p.qstride));

@@ -117,6 +117,7 @@ struct StepArgs {
   float u0, s0, s_inv, s_k;  // sponge: start, 1 / (nx - start), strength
   float lid[3];
   const float* omega_rot;
+  const float* phi;          // TLBM_FIELD solid fractions
   tlbm_fields fld;
   double* partials;          // report: per-block FP64 partial sums [TLBM_NSUMS][nblocks]
   uint32_t nblocks;
@@ -187,9 +188,24 @@ struct Box {
   }
 };
 
+// A per-cell solid fraction phi in [0, 1]: solid() is phi itself, so every link is a partial
+// bounce-back f = pulled + phi (bounced - pulled) and every cell takes the wall path.
+struct Field {
+  static constexpr bool kMoving = false;
+  __device__ static __forceinline__ float solid(const StepArgs& a, int x, int y, int z) {
+    return a.phi[(static_cast<uint32_t>(z) * a.ny + y) * static_cast<uint32_t>(a.nx) + x];
+  }
+  __device__ static __forceinline__ bool near(const StepArgs&, int, int, int) { return true; }
+  __device__ static __forceinline__ void wall_vel(const StepArgs&, float, float, float, float& u,
+                                                  float& v, float& w) {
+    u = v = w = 0.0f;
+  }
+};
+
 template <int OBS>
-using ObstacleT = std::conditional_t<OBS == TLBM_CYLINDER, Cylinder,
-                                     std::conditional_t<OBS == TLBM_SPHERE, Sphere, Box>>;
+using ObstacleT = std::conditional_t<
+    OBS == TLBM_CYLINDER, Cylinder,
+    std::conditional_t<OBS == TLBM_SPHERE, Sphere, std::conditional_t<OBS == TLBM_BOX, Box, Field>>>;
 
 template <int STORAGE>
 using StorageT = std::conditional_t<STORAGE == TLBM_F32, StoreF32,
@@ -484,9 +500,11 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
           t = 6.0f * kW[q] *
               (cx * (omega_rot * uwx) + cy * (omega_rot * uwy) + cz * (omega_rot * uwz));
         }
-        f[q] = b + t;
+        // lbm.py:428-429; with a 0/1 obstacle wall = 1 here, so f = b + t exactly
+        if constexpr (OBS == TLBM_FIELD) f[q] = f[q] + wall * (b - f[q] + t);
+        else f[q] = b + t;
         if constexpr (REPORT) {
-          const float hit = fluid * (2.0f * (kShift ? b + kW[o] : b) + t);
+          const float hit = wall * fluid * (2.0f * (kShift ? b + kW[o] : b) + t);
           mx += hit * static_cast<float>(kCX[o]);
           my += hit * static_cast<float>(kCY[o]);
           mz += hit * static_cast<float>(kCZ[o]);

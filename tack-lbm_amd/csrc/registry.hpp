@@ -9,7 +9,7 @@ namespace tlbm {
 
 using StepLaunch = hipError_t (*)(const StepArgs&, dim3 grid, dim3 block, hipStream_t stream);
 // Launchers of one (storage, collision) unit, indexed [obstacle][side][report].
-using StepTable = StepLaunch[3][3][2];
+using StepTable = StepLaunch[4][3][2];
 
 template <int C, int O, int S, bool R, int ST>
 hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t stream) {
@@ -18,7 +18,7 @@ hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t str
 }
 
 using BatchLaunch = hipError_t (*)(const BatchArgs&, dim3 grid, dim3 block, hipStream_t stream);
-using BatchTable = BatchLaunch[3][3][2];
+using BatchTable = BatchLaunch[4][3][2];
 
 template <int C, int O, int S, bool R, int ST>
 hipError_t launch_batch(const BatchArgs& b, dim3 grid, dim3 block, hipStream_t stream) {
@@ -28,7 +28,7 @@ hipError_t launch_batch(const BatchArgs& b, dim3 grid, dim3 block, hipStream_t s
 
 template <int C, int ST>
 void fill_table(StepTable& t, BatchTable& bt) {
-  static_for<0, 3>([&](auto o) {
+  static_for<0, 4>([&](auto o) {
     static_for<0, 3>([&](auto s) {
       constexpr int O = decltype(o)::value, S = decltype(s)::value;
       t[O][S][0] = &launch_step<C, O, S, false, ST>;

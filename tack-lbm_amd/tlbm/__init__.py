@@ -9,5 +9,6 @@ from . import _capi  # noqa: F401
 from . import autotune, batch  # noqa: F401
 from .lbm import (BGK, COLLISIONS, CX, CY, CZ, FIELDS, TRT, W, Box, Cylinder, D3Q19,  # noqa: F401
                   DeviceArray, Flow, HostBuffer, LidY, ManagedArray, NoHeat, PeriodicY, Quiet,
-                  Report, SlipY, SmagorinskyLES, Solver, Sphere, array, coarsen, set_launch,
+                  Report, SlipY, SmagorinskyLES, SolidField, Solver, Sphere, array, coarsen,
+                  set_launch,
                   decode_populations, empty, encode_populations, lbm_init, lbm_step, sync, zeros)

@@ -14,7 +14,7 @@ TLBM_OK = 0
 TLBM_NSUMS = 14
 
 BGK, TRT, LES = 0, 1, 2
-CYLINDER, SPHERE, BOX = 0, 1, 2
+CYLINDER, SPHERE, BOX, FIELD = 0, 1, 2, 3
 QUIET, REPORT = 0, 1
 PERIODIC_Y, SLIP_Y, LID_Y = 0, 1, 2
 F32, F16, BF16 = 0, 1, 2
@@ -31,7 +31,8 @@ class Params(ctypes.Structure):
                 ("omega", ctypes.c_float), ("tau0", ctypes.c_float), ("cs", ctypes.c_float),
                 ("magic", ctypes.c_float), ("obs", ctypes.c_float * 6), ("u0", ctypes.c_float),
                 ("sponge_start", ctypes.c_float), ("sponge_strength", ctypes.c_float),
-                ("lid", ctypes.c_float * 3), ("omega_rot", ctypes.c_void_p)]
+                ("lid", ctypes.c_float * 3), ("omega_rot", ctypes.c_void_p),
+                ("phi", ctypes.c_void_p)]
 
 
 FIELD_NAMES = ("ux", "uy", "uz", "p", "vmag", "r_rho", "r_ux", "r_uy", "r_uz", "r_ke",

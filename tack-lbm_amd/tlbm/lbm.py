@@ -298,6 +298,20 @@ class Box:
         return (self.x, self.y, self.z, self.hx, self.hy, self.hz)
 
 
+class SolidField:
+    """A per-cell solid fraction phi in [0, 1] (a DeviceArray of nx*ny*nz float32) in place of
+    an analytic obstacle: every link is a partial bounce-back f = pulled + phi (bounced -
+    pulled), and phi = 1 cells hold the rest equilibrium (lbm.py:403, 416, 428, 449 with
+    fractional `solid`).  No counterpart in the reference; the obstacle of the adjoint."""
+    code = _capi.FIELD
+
+    def __init__(self, phi):
+        self.phi = phi
+
+    def obs(self):
+        return (0.0,) * 6
+
+
 class NoHeat:
     """lbm.py:282-308: the thermal variant compiled out (``enabled = 0``)."""
     enabled = 0
@@ -399,6 +413,7 @@ def make_params(lat, phys, geo, flow, qstride=None):
     for i in range(3):
         p.lid[i] = lid[i]
     p.omega_rot = _ptr(flow.omega_rot)
+    p.phi = _ptr(getattr(geo, "phi", None))
     return p
 
 

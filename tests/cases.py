@@ -40,6 +40,13 @@ def random_fields(n, seed):
     return out
 
 
+def random_fraction(dims, seed=4):
+    """A seeded solid-fraction field: a third of the cells fluid (0), a fifth solid (1), the rest
+    fractional."""
+    rng = np.random.default_rng(seed)
+    return np.clip(rng.uniform(-0.5, 1.25, dims[0] * dims[1] * dims[2]), 0, 1).astype(np.float32)
+
+
 class Case:
     def __init__(self, dims, collision="bgk", obstacle="cylinder", side="periodic",
                  nu=0.02, u0=0.08, sponge=(0.5, 1.0), omega_rot=0.37, geometry=None, cs=0.16,
@@ -49,7 +56,10 @@ class Case:
         self.nu, self.u0, self.omega_rot = nu, u0, omega_rot
         self.sponge_start = sponge[0] * dims[0]
         self.sponge_strength = sponge[1]
-        self.geometry = GEOMETRIES[obstacle] if geometry is None else geometry
+        if obstacle == "field":
+            self.geometry = random_fraction(dims) if geometry is None else geometry
+        else:
+            self.geometry = GEOMETRIES[obstacle] if geometry is None else geometry
         self.cs, self.magic = cs, magic
 
     @property
@@ -60,8 +70,11 @@ class Case:
     def oracle(self):
         phys = {"bgk": lambda: O.BGK(self.nu), "trt": lambda: O.TRT(self.nu, self.magic, self.cs),
                 "les": lambda: O.LES(self.nu, self.cs)}[self.collision]()
-        geo = {"cylinder": O.Cylinder, "sphere": O.Sphere, "box": O.Box}[self.obstacle](
-            *self.geometry)
+        if self.obstacle == "field":
+            geo = O.Field(self.geometry, self.dims)
+        else:
+            geo = {"cylinder": O.Cylinder, "sphere": O.Sphere, "box": O.Box}[self.obstacle](
+                *self.geometry)
         side = O.LidY(*LID) if self.side == "lid" else O.SIDES[self.side]()
         flow = O.Flow(self.u0, self.sponge_start, self.sponge_strength, self.omega_rot, side)
         return phys, geo, flow
@@ -72,8 +85,11 @@ class Case:
         phys = {"bgk": lambda: tlbm.BGK(self.nu),
                 "trt": lambda: tlbm.TRT(self.nu, magic=self.magic, cs=self.cs),
                 "les": lambda: tlbm.SmagorinskyLES(self.nu, cs=self.cs)}[self.collision]()
-        geo = {"cylinder": tlbm.Cylinder, "sphere": tlbm.Sphere, "box": tlbm.Box}[self.obstacle](
-            *self.geometry)
+        if self.obstacle == "field":
+            geo = tlbm.SolidField(tlbm.array(self.geometry))
+        else:
+            geo = {"cylinder": tlbm.Cylinder, "sphere": tlbm.Sphere, "box": tlbm.Box}[
+                self.obstacle](*self.geometry)
         side = tlbm.LidY(*LID) if self.side == "lid" else {"periodic": tlbm.PeriodicY,
                                                           "slip": tlbm.SlipY}[self.side]()
         omega = tlbm.array(np.array([self.omega_rot], np.float32))

@@ -64,17 +64,17 @@ def test_kernel_lookup_covers_every_combination():
     seen = set()
     for st in range(3):
         for c in range(3):
-            for o in range(3):
+            for o in range(4):
                 for s in range(3):
                     for d in range(2):
                         h = ctypes.c_int(-1)
                         spec = _capi.Spec(c, o, d, s, st)
                         assert lib.tlbm_kernel_lookup(ctypes.byref(spec), ctypes.byref(h)) == 0
                         seen.add(h.value)
-    assert len(seen) == 162
+    assert len(seen) == 216
 
 
-@pytest.mark.parametrize("bad", [(3, 0, 0, 0, 0), (0, -1, 0, 0, 0), (0, 0, 2, 0, 0),
+@pytest.mark.parametrize("bad", [(3, 0, 0, 0, 0), (0, -1, 0, 0, 0), (0, 4, 0, 0, 0), (0, 0, 2, 0, 0),
                                  (0, 0, 0, 3, 0), (0, 0, 0, 0, 3)])
 def test_kernel_lookup_rejects_unknown_specs(bad):
     from tlbm import _capi

@@ -78,7 +78,7 @@ def _run_step_inner(case, f64, report, storage, seed, lat, phys, geo, flow):
     return got, ref, gfields, o_fields, sums.numpy.copy()
 
 
-COMBOS = [(c, o, s) for c in ("bgk", "trt", "les") for o in ("cylinder", "sphere", "box")
+COMBOS = [(c, o, s) for c in ("bgk", "trt", "les") for o in ("cylinder", "sphere", "box", "field")
           for s in ("periodic", "slip", "lid")]
 
 
@@ -149,9 +149,9 @@ def test_ragged_and_degenerate_grids(dims):
 @pytest.mark.parametrize("storage,bits", [("f16", 10), ("bf16", 7)])
 @pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
 @pytest.mark.parametrize("side", ["periodic", "slip", "lid"])
-@pytest.mark.parametrize("nx", [24, 130])
-def test_shifted_storage(storage, bits, collision, side, nx):
-    case = Case((nx, 12, 9), collision, "cylinder", side)
+@pytest.mark.parametrize("nx,obstacle", [(24, "cylinder"), (130, "cylinder"), (24, "field")])
+def test_shifted_storage(storage, bits, collision, side, nx, obstacle):
+    case = Case((nx, 12, 9), collision, obstacle, side)
     f = random_populations(case.n, seed=15)
     got, ref, *_ = _run_step(case, f, report=False, storage=storage)
     w = np.array(O.W)[:, None]
@@ -180,7 +180,7 @@ def test_solid_cells_hold_rest_equilibrium():
             assert not got.any()
 
 
-@pytest.mark.parametrize("obstacle", ["cylinder", "sphere", "box"])
+@pytest.mark.parametrize("obstacle", ["cylinder", "sphere", "box", "field"])
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
 def test_init(obstacle, storage):
     """lbm.py:383-394 with the seeded velocity noise."""

@@ -258,6 +258,28 @@ def test_fluid_mass_is_conserved_in_a_closed_slip_box():
 
 
 # --- sponge (lbm.py:446-456) --------------------------------------------------------------------------
+def test_fractional_solid_interpolates_streaming_and_bounce_back():
+    """phi = 0 streams, phi = 1 bounces back (the 0/1 obstacle), phi in between is the linear
+    blend of the two (lbm.py:428 read with a fractional `solid`)."""
+    dims = (6, 5, 4)
+    n = 120
+    f = O.W[:, None] + 0.01 * np.random.default_rng(11).standard_normal((19, n))
+    phys = O.BGK(1.0)
+    phys.omega = 0.0
+    flow = O.Flow(0.0, 1e9, 0.0)
+    x, y, z = O.coords(*dims)
+    box = O.Box(2.0, 2.0, 1.0, 0.6, 0.6, 0.6)
+    binary = box.solid(x, y, z)
+    a = O.step(dims, phys, box, flow, f)
+    b = O.step(dims, phys, O.Field(binary, dims), flow, f)
+    assert np.abs(a - b).max() < 1e-16
+    free = O.step(dims, phys, O.Field(np.zeros(n), dims), flow, f)
+    half = O.step(dims, phys, O.Field(0.5 * binary, dims), flow, f)
+    fluid = binary == 0
+    # in fluid cells with solid neighbours: half-way between streaming and bounce-back
+    assert np.allclose(half[:, fluid], 0.5 * (free[:, fluid] + a[:, fluid]), atol=1e-15)
+
+
 def test_sponge_at_full_strength_gives_inflow_equilibrium():
     dims = (8, 3, 2)
     f, _, _ = _near_eq(n=48, seed=7)
@@ -369,7 +391,7 @@ def _c_oracle():
 
 
 @pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
-@pytest.mark.parametrize("obstacle", ["cylinder", "sphere", "box"])
+@pytest.mark.parametrize("obstacle", ["cylinder", "sphere", "box", "field"])
 @pytest.mark.parametrize("side", ["periodic", "slip", "lid"])
 def test_c_oracle_matches_numpy_oracle(collision, obstacle, side):
     C_ = _c_oracle()
@@ -377,8 +399,9 @@ def test_c_oracle_matches_numpy_oracle(collision, obstacle, side):
     n = 24 * 12 * 9
     f = O.W[:, None] + 0.01 * np.random.default_rng(9).standard_normal((19, n))
     phys = {"bgk": O.BGK(0.02), "trt": O.TRT(0.01, 0.25, 0.1), "les": O.LES(0.01)}[collision]
+    phi = np.clip(np.random.default_rng(10).uniform(-0.5, 1.25, n), 0, 1).astype(np.float32)
     geo = {"cylinder": O.Cylinder(9.3, 6.2, 3.1), "sphere": O.Sphere(9.3, 6.2, 4.4, 3.1),
-           "box": O.Box(0.4, 6.2, 4.4, 2.6, 2.1, 2.4)}[obstacle]
+           "box": O.Box(0.4, 6.2, 4.4, 2.6, 2.1, 2.4), "field": O.Field(phi, dims)}[obstacle]
     side_obj = O.LidY(0.04, 0.01, -0.02) if side == "lid" else O.SIDES[side]()
     flow = O.Flow(0.08, 12.0, 1.0, 0.37, side_obj)
     d1, d2 = O.new_fields(n), O.new_fields(n)
