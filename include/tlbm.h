@@ -128,10 +128,12 @@ int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, fl
               uint64_t noise_seed, void* stream);
 
 /* lbm_step (lbm.py:397-456, 493-513): one fused pull-stream + bounce-back + collide + sponge
- * step from f_in to f_out (distinct buffers).  For a TLBM_REPORT handle, `fields` must hold the
+ * step from f_in to f_out (distinct buffers).  For a TLBM_REPORT handle, `fields` holds the
  * 13 report arrays (old ux, uy, uz, p are read, all 13 are written) and, when `sums` is not
  * NULL, TLBM_NSUMS FP64 sums are written to sums[0..13] (device, managed or pinned host
- * memory) by the same stream.  `fields` and `sums` are ignored by TLBM_QUIET handles.
+ * memory) by the same stream.  A report step may pass fields = NULL when sums is not NULL:
+ * it then writes only the sums, with the five residual sums zero.  `fields` and `sums` are
+ * ignored by TLBM_QUIET handles.
  * Replaces the host FP64 numpy sums of Solver.monitors (lbm.py:649-661). */
 int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* params,
               const tlbm_fields* fields, double* sums, void* stream);
@@ -144,6 +146,23 @@ int tlbm_copy(void* dst, const void* src, size_t bytes, void* stream);
 /* coarsen (lbm.py:516-522): dst[(k*my + j)*mx + i] = src[((k s)*ny + j s)*nx + i s]. */
 int tlbm_coarsen(const float* src, float* dst, int nx, int ny, int nz, int s, int mx, int my,
                  int mz, void* stream);
+
+/* ---- adjoint (no counterpart in the reference) --------------------------------------------
+ * One reverse step of the discrete adjoint of the BGK step with a TLBM_FIELD obstacle, periodic
+ * in x, y and z, FP32 populations, qstride = nx*ny*nz (params as for that forward step; phi is
+ * the solid fraction).  The objective is J = sum_t lam_t * (x-momentum exchanged with the solid
+ * at step t) (lbm.py:429-430 summed over cells).  Given mu_in = dJ/dg^{t+1} (g: the pulled and
+ * bounced populations before collision) and the states f_t, f_t1 of steps t and t + 1, writes
+ * mu_out = dJ/dg^t and adds dJ/dphi of the cells of the parameter block
+ * [origin, origin + size) (inside the lattice) to gphi (FP64, x fastest).  lam_t, lam_t1: the objective's weights
+ * of steps t and t + 1 (0 outside the averaging window; lam_t1 = 0 and mu_in = 0 for the last
+ * step).  final_links != 0 adds only the terms of step t + 1's links (mu_in = dJ/dg^0,
+ * f_t1 = the initial state): call it once after the reverse sweep.  Blocks may not overlap
+ * in flight (gphi is updated in place). */
+int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* f_t1,
+                      const float* mu_in, float* mu_out, float lam_t, float lam_t1,
+                      const int32_t* block_origin, const int32_t* block_size, double* gphi,
+                      int final_links, void* stream);
 
 /* ---- batched lattices (parameter sweeps) --------------------------------------------------
  * No counterpart in the reference, which steps one lattice per launch.  A batch is a set of

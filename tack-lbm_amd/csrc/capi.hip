@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint.hpp"
 #include "registry.hpp"
 #include "tlbm.h"
 
@@ -552,11 +553,14 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
   const uint32_t nblocks = grid.x * grid.y * grid.z;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (d.diag == TLBM_REPORT) {
-    if (!fields) return fail(TLBM_ERR_INVALID, "a report step needs the 13 field arrays");
-    const float* const* ptrs = reinterpret_cast<const float* const*>(fields);
-    for (int k = 0; k < 13; ++k)
-      if (!ptrs[k]) return fail(TLBM_ERR_INVALID, "report field %d is NULL", k);
-    a.fld = *fields;
+    if (!fields && !sums)
+      return fail(TLBM_ERR_INVALID, "a report step needs the field arrays, the sums or both");
+    if (fields) {
+      const float* const* ptrs = reinterpret_cast<const float* const*>(fields);
+      for (int k = 0; k < 13; ++k)
+        if (!ptrs[k]) return fail(TLBM_ERR_INVALID, "report field %d is NULL", k);
+      a.fld = *fields;
+    }
     if (sums) {
       double* ws = nullptr;
       if (int rc = workspace(sizeof(double) * TLBM_NSUMS * nblocks, &ws)) return rc;
@@ -779,6 +783,61 @@ int tlbm_graph_destroy(tlbm_graph* g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   delete g;
+  return TLBM_OK;
+}
+
+int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* f_t1,
+                      const float* mu_in, float* mu_out, float lam_t, float lam_t1,
+                      const int32_t* block_origin, const int32_t* block_size, double* gphi,
+                      int final_links, void* stream) {
+  if (int rc = check_params(params, TLBM_FIELD)) return rc;
+  if (!f_t1 || !mu_in || (!final_links && (!f_t || !mu_out)))
+    return fail(TLBM_ERR_INVALID, "adjoint buffers are NULL");
+  if (!final_links && mu_in == mu_out)
+    return fail(TLBM_ERR_INVALID, "mu_in and mu_out must be distinct buffers");
+  if (!block_origin || !block_size || !gphi)
+    return fail(TLBM_ERR_INVALID, "parameter block and its gradient buffer are required");
+  const uint64_t n = static_cast<uint64_t>(params->nx) * params->ny * params->nz;
+  if (static_cast<uint64_t>(params->qstride) != n)
+    return fail(TLBM_ERR_INVALID, "the adjoint needs qstride == nx*ny*nz");
+  tlbm::AdjArgs a{};
+  a.f_t = f_t;
+  a.f_t1 = f_t1;
+  a.mu_in = mu_in;
+  a.mu_out = mu_out;
+  a.phi = params->phi;
+  a.ns = n;
+  a.nx = params->nx;
+  a.ny = params->ny;
+  a.nz = params->nz;
+  a.omega = params->omega;
+  a.u0 = params->u0;
+  a.s0 = params->sponge_start;
+  a.s_inv = 1.0f / (static_cast<float>(params->nx) - params->sponge_start);
+  a.s_k = params->sponge_strength;
+  a.lam_t = lam_t;
+  a.lam_t1 = lam_t1;
+  const int32_t dims[3] = {params->nx, params->ny, params->nz};
+  for (int i = 0; i < 3; ++i) {
+    a.blk[i] = block_origin[i];
+    a.bn[i] = block_size[i];
+    if (block_size[i] <= 0 || block_origin[i] < 0 ||
+        static_cast<int64_t>(block_origin[i]) + block_size[i] > dims[i])
+      return fail(TLBM_ERR_INVALID, "parameter block [%d, %d) is not inside [0, %d) on axis %d",
+                  block_origin[i], block_origin[i] + block_size[i], dims[i], i);
+  }
+  a.gphi = gphi;
+  const uint64_t xblocks = (params->nx + 255) / 256;
+  a.xblocks = static_cast<uint32_t>(xblocks);
+  a.rows = static_cast<uint32_t>(static_cast<uint64_t>(params->ny) * params->nz);
+  const dim3 grid(static_cast<uint32_t>(xblocks * a.rows));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (final_links)
+    hipLaunchKernelGGL(tlbm::lbm_adjoint_kernel<true>, grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(tlbm::lbm_adjoint_kernel<false>, grid, dim3(256), 0, st, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "adjoint launch");
   return TLBM_OK;
 }
 

@@ -87,6 +87,9 @@ SIGNATURES = {
     "tlbm_graph_launch": (_c.c_int, [_vp, _vp]),
     "tlbm_graph_destroy": (_c.c_int, [_vp]),
     "tlbm_device_cus": (_c.c_int, [_c.POINTER(_c.c_int)]),
+    "tlbm_adjoint_step": (_c.c_int, [_c.POINTER(Params), _vp, _vp, _vp, _vp, _c.c_float,
+                                     _c.c_float, _c.POINTER(_c.c_int32),
+                                     _c.POINTER(_c.c_int32), _vp, _c.c_int, _vp]),
     "tlbm_device_name": (_c.c_int, [_c.c_char_p, _c.c_int]),
     "tlbm_coarsen": (_c.c_int, [_vp, _vp, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int,
                                 _c.c_int, _c.c_int, _vp]),

@@ -21,3 +21,23 @@ C2_SAMPLE_EVERY = 10
 C3 = dict(nx=512, ny=256, nz=256, collision="les", obstacle="sphere", reynolds=3000.0, u0=0.08,
           diameter=32.0, center=(96.0, 128.0, 128.0), lateral="lid", lid_u=(0.04, 0.0, 0.0),
           noise_amp=1e-3, noise_seed=0)
+
+# Adjoint: BGK 128x64x64 FP32, sphere radius 10 with a smoothed surface (eps 1.5) and a 32^3
+# porosity block around the nominal centre (32, 32, 32), Re 200 (nu from the nominal D = 20),
+# u0 = 0.08, 2000 steps from the uniform inflow state, checkpoints every 100; the objective is
+# the mean x force over the last 1000 steps.
+C5 = dict(nx=128, ny=64, nz=64, reynolds=200.0, u0=0.08, radius=10.0,
+          nominal_center=(32.0, 32.0, 32.0), block=32, steps=2000, window=1000,
+          checkpoint_every=100, eps=1.5)
+C5_SEED = 0
+C5_FD_STEP = 1e-3
+
+
+def c5_design(seed=C5_SEED, block=32):
+    """The seeded design point: radius 10, centre = nominal + U(-2, 2)^3 cells, porosity
+    U(0.01, 0.02) on the block (strictly positive, so that phi > 0 on every design cell)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    center = np.array(C5["nominal_center"]) + rng.uniform(-2.0, 2.0, 3)
+    porosity = 0.01 + 0.01 * rng.uniform(size=(block, block, block))
+    return C5["radius"], center, porosity
