@@ -553,6 +553,22 @@ def strouhal_fft(lift, sample_every, diameter, u0):
     return float(freq * diameter / u0)
 
 
+def strouhal_refined(lift, sample_every, diameter, u0, pad=16):
+    """Bin-free Strouhal number of a lift record (all of it; pass the window to use): the peak
+    of the Hann-windowed spectrum zero-padded ``pad`` times, refined by a parabola through the
+    log magnitudes of the peak bin and its two neighbours.  Not in the reference, whose
+    collision_check.py:27-31 estimator quantises St to D / (u0 * sample_every * N/2)."""
+    cl = np.asarray(lift, np.float64)
+    cl = cl - cl.mean()
+    n = len(cl)
+    spec = np.abs(np.fft.rfft(cl * np.hanning(n), pad * n))
+    k = int(np.argmax(spec[1:-1])) + 1
+    a, b, c = np.log(spec[k - 1:k + 2] + 1e-300)
+    shift = 0.5 * (a - c) / (a - 2.0 * b + c) if a - 2.0 * b + c != 0 else 0.0
+    freq = (k + shift) / (pad * n * sample_every)
+    return float(freq * diameter / u0)
+
+
 def _selfcheck():
     assert abs(W.sum() - 1.0) < 1e-15
     assert math.isclose(float((W * CX * CX).sum()), 1 / 3)

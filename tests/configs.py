@@ -41,3 +41,12 @@ def c5_design(seed=C5_SEED, block=32):
     center = np.array(C5["nominal_center"]) + rng.uniform(-2.0, 2.0, 3)
     porosity = 0.01 + 0.01 * rng.uniform(size=(block, block, block))
     return C5["radius"], center, porosity
+
+# collision_check.py's Strouhal case: the default Solver (lbm.py:548-561: 384x128x64, D = 24,
+# u0 = 0.08, cylinder at (0.22 nx, 0.5 ny + 1.7), sponge from 0.75 nx), TRT at Re 300, monitors
+# every 50 steps (collision_check.py:12-20); the published St = 0.200 (README.md:66-67).
+ST = dict(nx=384, ny=128, nz=64, collision="trt", obstacle="cylinder", reynolds=300.0, u0=0.08,
+          diameter=24.0)
+ST_EVERY = 50
+ST_STEPS = 16000
+ST_WINDOW = (8000, 16000)  # shedding saturated

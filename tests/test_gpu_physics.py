@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-from configs import C1, C1_STEPS
+from configs import C1, C1_STEPS, C2, C2_SAMPLE_EVERY, C2_STEPS, ST, ST_EVERY, ST_STEPS, ST_WINDOW
 
 pytestmark = pytest.mark.gpu
 
@@ -88,3 +88,43 @@ def test_c1_bgk_cylinder_against_fp64_oracle(storage):
           f"drag {d[-1]:.5f} (oracle {d_ref[-1]:.5f}) max rel {err[2]:.2e}  lift max {err[3]:.2e}")
     for e, b in zip(err, C1_BOUNDS[storage]):
         assert e <= b, (err, C1_BOUNDS[storage])
+
+
+@pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
+def test_re300_strouhal_with_the_collision_check_estimator(collision):
+    """collision_check.py:12-31 on the default Solver at Re 300, run for 12000 steps: its FFT
+    arg-max estimator (bin width D / (u0 * 50 * 120) = 0.05 in St) gives the published 0.200
+    (README.md:66-67) for all three collision models.  The bin-free estimate of the same record
+    (shedding saturated, steps 8000-16000) is 0.22: the published value is that bin."""
+    from oracle import lbm_oracle as O
+    cfg = dict(ST, collision=collision)
+    d, lift, _ = _run_gpu(cfg, ST_STEPS, ST_EVERY, "f32")
+    st_bin = O.strouhal_fft(lift[:12000 // ST_EVERY], ST_EVERY, cfg["diameter"], cfg["u0"])
+    a, b = (w // ST_EVERY for w in ST_WINDOW)
+    st = O.strouhal_refined(lift[a:b], ST_EVERY, cfg["diameter"], cfg["u0"])
+    print(f"\nRe 300 {collision}: St (collision_check estimator, 12000 steps) {st_bin:.3f}; "
+          f"bin-free St {st:.4f}; mean drag {d[a:b].mean():.4f}")
+    assert abs(st_bin - 0.200) < 1e-9
+    assert 0.2 < st < 0.25
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_re300_strouhal_and_drag_against_fp64_record(storage):
+    """The same case (TRT) against the FP64 C-oracle record tests/golden/st_trt_re300_lift.npz:
+    bin-free St and mean drag over steps 8000-16000 within 0.5 % (FP32) and 1 % (FP16, BF16)."""
+    from oracle import lbm_oracle as O
+    path = os.path.join(ROOT, "tests", "golden", "st_trt_re300_lift.npz")
+    if not os.path.exists(path):
+        pytest.skip("tests/golden/st_trt_re300_lift.npz not generated")
+    ref = np.load(path)
+    d, lift, _ = _run_gpu(ST, ST_STEPS, ST_EVERY, storage)
+    a, b = (w // ST_EVERY for w in ST_WINDOW)
+    assert np.array_equal(ref["step"][a:b], np.arange(a + 1, b + 1) * ST_EVERY)
+    st = O.strouhal_refined(lift[a:b], ST_EVERY, ST["diameter"], ST["u0"])
+    st_ref = O.strouhal_refined(ref["lift"][a:b], ST_EVERY, ST["diameter"], ST["u0"])
+    dm, dm_ref = d[a:b].mean(), ref["drag"][a:b].mean()
+    bound = 0.005 if storage == "f32" else 0.01
+    print(f"\nRe 300 TRT {storage}: St {st:.4f} (FP64 {st_ref:.4f}, {st / st_ref - 1:+.2e}); "
+          f"mean drag {dm:.5f} (FP64 {dm_ref:.5f}, {dm / dm_ref - 1:+.2e})")
+    assert abs(st / st_ref - 1) <= bound
+    assert abs(dm / dm_ref - 1) <= bound
