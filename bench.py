@@ -15,9 +15,11 @@ slab of the same workload, single-threaded.
 
 sweep: the batched parameter sweep (64 uneven TRT channels, one persistent launch per step from
 a hipGraph), FP32 and FP16 storage, reported under "sweep".
+adjoint: the gradient workload (BGK 128x64x64, 2000 steps checkpointed every 100), forward and
+backward pass times and the adjoint kernel against the forward step kernel, under "adjoint".
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
-                       [--storage f32,f16,bf16] [--block N] [--calibrate]
+                       [--no-adjoint] [--storage f32,f16,bf16] [--block N] [--calibrate]
 """
 
 import argparse
@@ -224,6 +226,47 @@ def time_sweep(storage, steps=5000, graph_steps=100, seq_steps=20):
             "drag_range": [min(m["drag"] for m in mons), max(m["drag"] for m in mons)]}
 
 
+def time_adjoint(reps=100):
+    """The adjoint workload: BGK 128x64x64 FP32, smoothed sphere (radius 10, centre seeded
+    within +-2 cells of (32, 32, 32)) times a 32^3 porosity block, Re 200, 2000 steps with
+    checkpoints every 100, d(mean drag over the last 1000 steps)/d(radius, centre, porosity).
+    Reports the whole forward and backward passes (HIP events), and the adjoint kernel against
+    the forward quiet step kernel on the same lattice (`reps` back-to-back launches each)."""
+    import numpy as np
+    import tlbm  # noqa: F401
+    from tlbm.adjoint import DragAdjoint
+    from tlbm.adjoint import _elapsed, _event
+    rng = np.random.default_rng(0)
+    center = np.array([32.0, 32.0, 32.0]) + rng.uniform(-2.0, 2.0, 3)
+    porosity = 0.01 + 0.01 * rng.uniform(size=(32, 32, 32))
+    prob = DragAdjoint(nx=128, ny=64, nz=64, reynolds=200.0, u0=0.08, radius=10.0,
+                       nominal_center=(32.0, 32.0, 32.0), block=32, steps=2000, window=1000,
+                       checkpoint_every=100, eps=1.5)
+    prob.forward(10.0, center, porosity)
+    prob.backward()  # warm-up (allocates the segment buffers)
+    J = prob.forward(10.0, center, porosity)
+    g = prob.backward()
+    fwd, bwd = prob.forward_ms, prob.backward_ms
+    run = prob.run
+    e0 = _event(run.stream)
+    for _ in range(reps):
+        run._step(run.ckpt[0], run._seg[0])
+    step_ms = _elapsed(e0, _event(run.stream)) / reps
+    e0 = _event(run.stream)
+    for _ in range(reps):
+        run._adjoint(run.ckpt[0], run._seg[0], run._mu[0], run._mu[1], 1e-3, 1e-3)
+    adj_ms = _elapsed(e0, _event(run.stream)) / reps
+    prob.close()
+    steps = 2000
+    return {"workload": "D3Q19 BGK 128x64x64 FP32, smoothed sphere r10 + 32^3 porosity, Re 200, "
+                        "2000 steps, checkpoints every 100, J = mean drag over the last 1000",
+            "J": J, "dJ_dradius": g["radius"], "dJ_dcenter": [float(v) for v in g["center"]],
+            "forward_ms": fwd, "backward_ms": bwd, "backward_over_forward": bwd / fwd,
+            "forward_ms_per_step": fwd / steps, "backward_ms_per_step": bwd / steps,
+            "adjoint_kernel_ms": adj_ms, "forward_step_kernel_ms": step_ms,
+            "adjoint_kernel_over_step_kernel": adj_ms / step_ms}
+
+
 def cpu_baseline(max_seconds=20.0):
     """The NumPy float32 restatement on a 512x256x4 slab of the workload, one core."""
     import numpy as np
@@ -256,6 +299,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0 = default)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched parameter sweep")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--calibrate", action="store_true",
                     help="also time a plain 1 GiB copy (counter calibration, streaming ceiling)")
     args = ap.parse_args()
@@ -277,6 +321,13 @@ def main():
             print(f"[bench] sweep {st}: {sweep[st]['mlups']:.0f} MLUPS over {sweep[st]['cells']} "
                   f"cells in {sweep[st]['domains']} domains (one launch per domain: "
                   f"{sweep[st]['one_launch_per_domain_mlups']:.0f})", file=sys.stderr, flush=True)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = time_adjoint()
+        print(f"[bench] adjoint: forward {adjoint['forward_ms']:.1f} ms, backward "
+              f"{adjoint['backward_ms']:.1f} ms ({adjoint['backward_over_forward']:.2f}x); "
+              f"adjoint kernel {adjoint['adjoint_kernel_ms'] * 1e3:.1f} us vs step kernel "
+              f"{adjoint['forward_step_kernel_ms'] * 1e3:.1f} us", file=sys.stderr, flush=True)
     copy = calibrate_copy() if args.calibrate else None
     if copy:
         print(f"[bench] copy kernel: {copy['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -315,6 +366,9 @@ def main():
     if sweep:
         line["sweep"] = {st: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                          for st, r in sweep.items()}
+    if adjoint:
+        line["adjoint"] = {k: (round(v, 6) if isinstance(v, float) else v)
+                           for k, v in adjoint.items()}
     if copy:
         line["copy_calibration"] = {k: round(v, 4) if isinstance(v, float) else v
                                     for k, v in copy.items()}
