@@ -147,6 +147,25 @@ int tlbm_copy(void* dst, const void* src, size_t bytes, void* stream);
 int tlbm_coarsen(const float* src, float* dst, int nx, int ny, int nz, int s, int mx, int my,
                  int mz, void* stream);
 
+/* ---- derived fields of the output grid (derived.py:48-93) ----------------------------------
+ * Grid of mx x my x mz points (x fastest), spacing h, inv_h = 1 / h; its cells are
+ * (mx - 1)(my - 1)(mz - 1), indexed over their lower corner.  Arrays are device memory. */
+
+/* The velocity gradient at every point (derived.py:52-89, the solver's gradient): central
+ * differences, one-sided at the grid's faces, zero along an axis of one point.  jac holds 9
+ * planes of mx*my*mz floats: jac[(3 c + d) * m + p] = d u_c / d x_d at point p. */
+int tlbm_velocity_gradient(const float* ux, const float* uy, const float* uz, float* jac, int mx,
+                           int my, int mz, float inv_h, void* stream);
+
+/* |curl u| and the Q-criterion at every cell centre (the default derived fields,
+ * derived.py:26-29).  basis = 0: the cell's gradient is the mean of its 8 corners' point
+ * gradients (jac from tlbm_velocity_gradient); basis = 1: the trilinear cell's gradient at its
+ * centre, the mean of the four edge differences per axis (velocity arrays).  Either output may
+ * be NULL. */
+int tlbm_vortex_fields(const float* ux, const float* uy, const float* uz, const float* jac,
+                       float* vorticity, float* qcrit, int mx, int my, int mz, float inv_h,
+                       int basis, void* stream);
+
 /* ---- adjoint (no counterpart in the reference) --------------------------------------------
  * One reverse step of the discrete adjoint of the BGK step with a TLBM_FIELD obstacle, periodic
  * in x, y and z, FP32 populations, qstride = nx*ny*nz (params as for that forward step; phi is

@@ -37,6 +37,20 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(TLBM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
+}  // namespace
+
+// used by the other translation units of the library (derived.hip)
+namespace tlbm_detail {
+__attribute__((visibility("hidden"))) int fail_message(const char* msg) {
+  return fail(TLBM_ERR_INVALID, "%s", msg);
+}
+__attribute__((visibility("hidden"))) int fail_hip(hipError_t e, const char* what) {
+  return hip_fail(e, what);
+}
+}  // namespace tlbm_detail
+
+namespace {
+
 #define TLBM_HIP(call)                                      \
   do {                                                      \
     hipError_t e_ = (call);                                 \
