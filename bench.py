@@ -17,9 +17,11 @@ sweep: the batched parameter sweep (64 uneven TRT channels, one persistent launc
 a hipGraph), FP32 and FP16 storage, reported under "sweep".
 adjoint: the gradient workload (BGK 128x64x64, 2000 steps checkpointed every 100), forward and
 backward pass times and the adjoint kernel against the forward step kernel, under "adjoint".
+derived: vorticity magnitude and Q-criterion of the 512x256x256 output grid, under "derived".
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
-                       [--no-adjoint] [--storage f32,f16,bf16] [--block N] [--calibrate]
+                       [--no-adjoint] [--no-derived] [--storage f32,f16,bf16] [--block N]
+                       [--calibrate]
 """
 
 import argparse
@@ -267,6 +269,43 @@ def time_adjoint(reps=100):
             "adjoint_kernel_over_step_kernel": adj_ms / step_ms}
 
 
+def time_derived(reps=20):
+    """The default derived fields (vorticity magnitude and Q-criterion at the cell centres, the
+    solver's gradient) of the headline workload's output grid at stride 1, against one quiet
+    FP32 step: HIP events over `reps` evaluations (kernels only, no host copy)."""
+    import tlbm
+    from tlbm._capi import check, lib
+    from tlbm.derived import DerivedFields
+    L = lib()
+    stream = ctypes.c_void_p()
+    check(L.tlbm_stream_create(ctypes.byref(stream)), "stream create")
+    s = make_solver("f32", stream.value)
+    for _ in range(5):
+        s.step()
+    s.step(report=True)
+    d = DerivedFields(s)
+    d.compute()
+    ev = Events(L)
+    e0, e1, e2 = ev.new(), ev.new(), ev.new()
+    tlbm.sync(stream.value)
+    check(L.tlbm_event_record(e0, stream), "record")
+    for _ in range(reps):
+        d.compute()
+    check(L.tlbm_event_record(e1, stream), "record")
+    for _ in range(reps):
+        s.step()
+    check(L.tlbm_event_record(e2, stream), "record")
+    tlbm.sync(stream.value)
+    ms, step_ms = ev.elapsed(e0, e1) / reps, ev.elapsed(e1, e2) / reps
+    m, cells = d.n, d.cells
+    # gradient: 3 velocity reads + 9 writes per point; cells: 9 gradient reads + 2 writes per cell
+    algorithmic = 4 * (12 * m + 11 * cells)
+    del d, s
+    L.tlbm_stream_destroy(stream)
+    return {"grid": f"{NX}x{NY}x{NZ} points", "ms": ms, "step_ms": step_ms,
+            "achieved_gbs": algorithmic / (ms * 1e-3) / 1e9, "algorithmic_bytes": algorithmic}
+
+
 def cpu_baseline(max_seconds=20.0):
     """The NumPy float32 restatement on a 512x256x4 slab of the workload, one core."""
     import numpy as np
@@ -300,6 +339,7 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="threads per block (0 = default)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched parameter sweep")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
+    ap.add_argument("--no-derived", action="store_true", help="skip the derived-field timing")
     ap.add_argument("--calibrate", action="store_true",
                     help="also time a plain 1 GiB copy (counter calibration, streaming ceiling)")
     args = ap.parse_args()
@@ -328,6 +368,11 @@ def main():
               f"{adjoint['backward_ms']:.1f} ms ({adjoint['backward_over_forward']:.2f}x); "
               f"adjoint kernel {adjoint['adjoint_kernel_ms'] * 1e3:.1f} us vs step kernel "
               f"{adjoint['forward_step_kernel_ms'] * 1e3:.1f} us", file=sys.stderr, flush=True)
+    derived = None
+    if not args.no_derived:
+        derived = time_derived()
+        print(f"[bench] derived fields: {derived['ms']:.3f} ms ({derived['achieved_gbs']:.0f} "
+              f"GB/s) vs one step {derived['step_ms']:.3f} ms", file=sys.stderr, flush=True)
     copy = calibrate_copy() if args.calibrate else None
     if copy:
         print(f"[bench] copy kernel: {copy['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -369,6 +414,9 @@ def main():
     if adjoint:
         line["adjoint"] = {k: (round(v, 6) if isinstance(v, float) else v)
                            for k, v in adjoint.items()}
+    if derived:
+        line["derived"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                           for k, v in derived.items()}
     if copy:
         line["copy_calibration"] = {k: round(v, 4) if isinstance(v, float) else v
                                     for k, v in copy.items()}
