@@ -58,6 +58,13 @@ def mirror_y():
 OPP = opposite()
 MIRY = mirror_y()
 
+# D3Q7 for temperature (lbm.py:20-25): rest plus the six faces, c_s^2 = 1/4
+CX7 = np.array([0, 1, -1, 0, 0, 0, 0], np.int64)
+CY7 = np.array([0, 0, 0, 1, -1, 0, 0], np.int64)
+CZ7 = np.array([0, 0, 0, 0, 0, 1, -1], np.int64)
+W7 = np.array([1 / 4] + [1 / 8] * 6, np.float64)
+OPP7 = np.array([0, 2, 1, 4, 3, 6, 5], np.int64)
+
 
 def _c(v, dt):
     return dt.type(v)
@@ -178,6 +185,10 @@ class Cylinder:
         dy = _f32(y) - self.y
         return (dx * dx + dy * dy < self.r * self.r).astype(np.float32)
 
+    def sink(self, x, y, z):
+        """lbm.py:156, 185, 214: the whole obstacle holds the solid temperature."""
+        return self.solid(x, y, z)
+
     def wall_vel(self, x, y, z, dt):
         r = _c(self.r, dt)
         return (-(y - _c(self.y, dt)) / r, (x - _c(self.x, dt)) / r, np.zeros_like(x))
@@ -195,6 +206,10 @@ class Sphere:
         dy = _f32(y) - self.y
         dz = _f32(z) - self.z
         return (dx * dx + dy * dy + dz * dz < self.r * self.r).astype(np.float32)
+
+    def sink(self, x, y, z):
+        """lbm.py:156, 185, 214: the whole obstacle holds the solid temperature."""
+        return self.solid(x, y, z)
 
     def wall_vel(self, x, y, z, dt):
         zero = np.zeros_like(x)
@@ -215,6 +230,10 @@ class Box:
                        np.abs(_f32(z) - self.z) - self.hz)
         return (d < np.float32(0.0)).astype(np.float32)
 
+    def sink(self, x, y, z):
+        """lbm.py:156, 185, 214: the whole obstacle holds the solid temperature."""
+        return self.solid(x, y, z)
+
     def wall_vel(self, x, y, z, dt):
         zero = np.zeros_like(x)
         return zero, zero, zero
@@ -232,6 +251,43 @@ class Field:
 
     def solid(self, x, y, z):
         return self.phi[(np.asarray(z) * self.ny + np.asarray(y)) * self.nx + np.asarray(x)]
+
+    def sink(self, x, y, z):
+        """lbm.py:156, 185, 214: the whole obstacle holds the solid temperature."""
+        return self.solid(x, y, z)
+
+    def wall_vel(self, x, y, z, dt):
+        zero = np.zeros_like(x)
+        return zero, zero, zero
+
+
+class HeatSink:
+    """lbm.py:231-268: a finned heat sink on the floor of a duct.  The base plate and the fins
+    (parallel to x, ``fin_thickness`` cells every ``fin_pitch`` along z) are the sink; the duct's
+    floor and ceiling rows (y = 0 and y = ny - 1) are adiabatic walls.  Resting walls."""
+    kind = "heatsink"
+
+    def __init__(self, ny, x0, length, z0, width, base, fin_height, fin_thickness, fin_pitch):
+        self.ny = int(ny)
+        self.x0, self.x1 = np.float32(x0), np.float32(x0 + length)
+        self.z0, self.z1 = np.float32(z0), np.float32(z0 + width)
+        self.y_base = np.float32(1.0 + base)
+        self.y_top = np.float32(1.0 + base + fin_height)
+        self.fin_thickness = int(fin_thickness)
+        self.fin_pitch = int(fin_pitch)
+
+    def sink(self, x, y, z):
+        fx, fy, fz = _f32(x), _f32(y), _f32(z)
+        inside = (fx >= self.x0) & (fx < self.x1) & (fz >= self.z0) & (fz < self.z1) & (fy >= 1.0)
+        base = fy < self.y_base
+        fin = (fy < self.y_top) & ((np.asarray(z) - int(self.z0)) % self.fin_pitch
+                                    < self.fin_thickness)
+        return (inside & (base | fin)).astype(np.float32)
+
+    def solid(self, x, y, z):
+        y = np.asarray(y)
+        wall = ((y <= 0) | (y >= self.ny - 1)).astype(np.float32)
+        return np.maximum(wall, self.sink(x, y, z))
 
     def wall_vel(self, x, y, z, dt):
         zero = np.zeros_like(x)
@@ -377,6 +433,11 @@ def step(dims, phys, geo, flow, f_in, diag=None):
     ``f_in`` is ``(19, n)`` in the arithmetic dtype.  ``diag`` (a dict of the 13 fields) makes it
     the report variant (lbm.py:493-513): the fields are updated in place.  Returns ``f_out``.
     """
+    return _step_core(dims, phys, geo, flow, f_in, diag)[0]
+
+
+def _step_core(dims, phys, geo, flow, f_in, diag=None):
+    """``step``, also returning the cell quantities the temperature step reads."""
     nx, ny, nz = dims
     n = nx * ny * nz
     dt = f_in.dtype
@@ -462,7 +523,83 @@ def step(dims, phys, geo, flow, f_in, diag=None):
         diag["fy"][:] = my
         diag["fz"][:] = mz
         diag["_rho"] = rho
-    return f_out
+    aux = dict(here=here, fluid=fluid, tau=tau, sigma=sigma, ux=ux, uy=uy, uz=uz, ux_s=ux_s,
+               uy_s=uy_s, uz_s=uz_s)
+    return f_out, aux
+
+
+# --- the temperature variant: lbm.py:283-320, 457-492 ---------------------------------------------
+class Heat:
+    """lbm.py:283-320 with ``enabled = 1``: D3Q7 advection-diffusion of the temperature with
+    diffusivity alpha + max(0, (tau - 1/2)/3 - nu) / pr_t (molecular plus subgrid, turbulent
+    Prandtl number 0.85).  ``t_solid``: the temperature of every solid voxel (n)."""
+    Q = 7
+    pr_t = 0.85
+
+    def __init__(self, alpha, nu, t_solid):
+        self.alpha, self.nu = alpha, nu
+        self.t_solid = np.asarray(t_solid)
+
+    @staticmethod
+    def geq(q, temp, ux, uy, uz):
+        """lbm.py:300-303: w_q T (1 + 4 c_q . u)."""
+        cu = float(CX7[q]) * ux + float(CY7[q]) * uy + float(CZ7[q]) * uz
+        return W7[q] * temp * (1.0 + 4.0 * cu)
+
+    def diffusivity(self, tau):
+        """lbm.py:305-308."""
+        return self.alpha + np.maximum(0.0, (tau - 0.5) / 3.0 - self.nu) / self.pr_t
+
+
+def step_heat(dims, phys, geo, flow, f_in, g_in, heat, diag=None, heat_out=None):
+    """``step`` with the temperature variant (lbm.py:457-492), FP64.  Returns (f_out, g_out).
+
+    ``g_in``: (7, n).  The temperature populations are pulled periodically in x, y and z; a link
+    from a solid voxel bounces back (adiabatic) or, from a sink voxel, anti-bounces to its
+    temperature t_solid (lbm.py:468-473).  With ``diag``, ``heat_out`` (a dict with "temp" and
+    "q_solid", n each) receives the cell temperature and, for every wetted sink voxel, the sum of
+    its fluid neighbours' temperatures (lbm.py:482-492; q_solid is added to, not cleared).
+    """
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    f_out, aux = _step_core(dims, phys, geo, flow, f_in, diag)
+    g_in = np.asarray(g_in, np.float64).reshape(7, n)
+    x, y, z = coords(nx, ny, nz)
+    idx = np.arange(n)
+    ts = heat.t_solid.astype(np.float64)
+    g = np.empty((7, n))
+    for q in range(7):
+        xs = (x - CX7[q] + nx) % nx
+        ys = (y - CY7[q] + ny) % ny
+        zs = (z - CZ7[q] + nz) % nz
+        nidx = (zs * ny + ys) * nx + xs
+        wall = geo.solid(xs, ys, zs).astype(np.float64)
+        hot = geo.sink(xs, ys, zs).astype(np.float64)
+        pulled = g_in[q, nidx]
+        bounced = g_in[OPP7[q], idx]
+        at_wall = bounced * (1.0 - 2.0 * hot) + hot * 2.0 * W7[q] * ts[nidx]
+        g[q] = pulled + wall * (at_wall - pulled)
+    temp = g.sum(axis=0)
+    alpha = heat.diffusivity(aux["tau"])
+    omega_g = 1.0 / (4.0 * alpha + 0.5)
+    sigma = aux["sigma"]
+    temp_s = (1.0 - sigma) * temp
+    g_out = np.empty((7, n))
+    for q in range(7):
+        eq = heat.geq(q, temp, aux["ux"], aux["uy"], aux["uz"])
+        post = g[q] - omega_g * (g[q] - eq)
+        g_out[q] = heat.geq(q, temp_s, aux["ux_s"], aux["uy_s"], aux["uz_s"]) + \
+            (1.0 - sigma) * (post - eq)
+    if diag is not None and heat_out is not None:
+        heat_out["temp"][:] = aux["fluid"] * temp + aux["here"] * ts
+        for q in range(1, 7):
+            xs = (x - CX7[q] + nx) % nx
+            ys = (y - CY7[q] + ny) % ny
+            zs = (z - CZ7[q] + nz) % nz
+            nidx = (zs * ny + ys) * nx + xs
+            hot = geo.sink(xs, ys, zs).astype(np.float64)
+            np.add.at(heat_out["q_solid"], nidx, aux["fluid"] * hot * temp)
+    return f_out, g_out
 
 
 def monitors(fields, n, interval, u0, diameter, nz):
@@ -497,6 +634,9 @@ def make_obstacle(kind, nx, ny, nz, diameter, center=None):
         return Sphere(x0, y0, z0, r)
     if kind == "box":
         return Box(x0, y0, z0, r, r, 0.3 * nz)
+    if kind == "heatsink":                                        # lbm.py:635-637
+        return HeatSink(ny, int(0.2 * nx), int(0.3 * nx), int(0.125 * nz), int(0.75 * nz),
+                        round(diameter / 2), int(0.45 * ny), round(diameter / 3), round(diameter))
     raise KeyError(kind)
 
 
