@@ -32,11 +32,13 @@ extern "C" {
 
 /* collision models: lbm.py:544 COLLISIONS = {"bgk": BGK, "trt": TRT, "les": SmagorinskyLES} */
 enum { TLBM_BGK = 0, TLBM_TRT = 1, TLBM_LES = 2 };
-/* obstacles: lbm.py:144-227 (Cylinder, Sphere, Box); TLBM_FIELD has no counterpart in the
- * reference: a per-cell solid fraction phi in [0, 1] (tlbm_params.phi), used as the `solid`
- * value of lbm.py:403 and 416 -- partial bounce-back f = pulled + phi (bounced - pulled) on each
- * link, the smoothed, differentiable obstacle of the adjoint. */
-enum { TLBM_CYLINDER = 0, TLBM_SPHERE = 1, TLBM_BOX = 2, TLBM_FIELD = 3 };
+/* obstacles: lbm.py:144-227 (Cylinder, Sphere, Box) and lbm.py:231-268 (HeatSink: a finned
+ * sink on the floor of a duct whose y = 0 and y = ny - 1 rows are walls).  TLBM_FIELD has no
+ * counterpart in the reference: a per-cell solid fraction phi in [0, 1] (tlbm_params.phi), used
+ * as the `solid` value of lbm.py:403 and 416 -- partial bounce-back
+ * f = pulled + phi (bounced - pulled) on each link, the smoothed, differentiable obstacle of
+ * the adjoint. */
+enum { TLBM_CYLINDER = 0, TLBM_SPHERE = 1, TLBM_BOX = 2, TLBM_FIELD = 3, TLBM_HEATSINK = 4 };
 /* diagnostics variants: lbm.py:325-339 (Quiet.enabled = 0, Report.enabled = 1) */
 enum { TLBM_QUIET = 0, TLBM_REPORT = 1 };
 /* lateral y sides: lbm.py:342-369 (PeriodicY, SlipY); TLBM_LID_Y has no counterpart in the
@@ -57,6 +59,8 @@ typedef struct {
   int32_t diag;      /* TLBM_QUIET / TLBM_REPORT                  */
   int32_t side;      /* TLBM_PERIODIC_Y / TLBM_SLIP_Y / TLBM_LID_Y */
   int32_t storage;   /* TLBM_F32 / TLBM_F16 / TLBM_BF16           */
+  int32_t heat;      /* 0: NoHeat; 1: Heat, the D3Q7 temperature variant (lbm.py:279-319,
+                        457-492), TLBM_F32 storage only; such handles step with tlbm_step_heat */
 } tlbm_spec;
 
 /* Runtime arguments of one step (the reference's plugin instance attributes). */
@@ -68,8 +72,10 @@ typedef struct {
   float tau0;              /* TRT / LES: 3 nu + 1/2, lbm.py:110, 131                    */
   float cs;                /* Smagorinsky constant (TRT.cs, SmagorinskyLES.cs = 0.16)    */
   float magic;             /* TRT magic parameter (TRT.magic = 0.1875), lbm.py:106       */
-  float obs[6];            /* cylinder: x, y, r; sphere: x, y, z, r;
-                              box: x, y, z, hx, hy, hz  (lbm.py:146, 174, 203-205)       */
+  float obs[8];            /* cylinder: x, y, r; sphere: x, y, z, r;
+                              box: x, y, z, hx, hy, hz  (lbm.py:146, 174, 203-205);
+                              heat sink: x0, x1, z0, z1, y_base, y_top, fin_thickness,
+                              fin_pitch (lbm.py:239-246; integers except the y levels) */
   float u0;                /* Flow.u0, lbm.py:375                                        */
   float sponge_start;      /* Flow.sponge_start                                          */
   float sponge_strength;   /* Flow.sponge_strength                                       */
@@ -146,6 +152,30 @@ int tlbm_copy(void* dst, const void* src, size_t bytes, void* stream);
 /* coarsen (lbm.py:516-522): dst[(k*my + j)*mx + i] = src[((k s)*ny + j s)*nx + i s]. */
 int tlbm_coarsen(const float* src, float* dst, int nx, int ny, int nz, int s, int mx, int my,
                  int mz, void* stream);
+
+/* ---- the temperature variant (lbm.py:279-319, 457-492) ------------------------------------
+ * D3Q7 advection-diffusion of the temperature inside the same step: g is pulled periodically
+ * in x, y and z; a link from a solid voxel bounces back (adiabatic) and one from a sink voxel
+ * (the whole obstacle, or the heat sink's base and fins) anti-bounces to its t_solid
+ * (lbm.py:468-473); BGK relaxation with omega_g = 1 / (4 alpha_eff + 1/2),
+ * alpha_eff = alpha + max(0, (tau - 1/2) / 3 - nu) / pr_t, tau the step's relaxation time;
+ * the flow's sponge blend applies to the temperature too. */
+typedef struct {
+  const float* t_solid; /* n: the temperature of each solid voxel (Solver.set_wall_temperature) */
+  float* q_solid;       /* n: report steps add, atomically, fluid * temp of every fluid cell to
+                           each sink voxel it faces (lbm.py:482-492); the caller clears it    */
+  float* temp;          /* n: report steps write fluid * temp + solid * t_solid (lbm.py:483)  */
+  float alpha;          /* molecular diffusivity, nu / prandtl (lbm.py:597)                   */
+  float nu;             /* molecular viscosity                                                */
+  float pr_t;           /* turbulent Prandtl number (Heat.pr_t = 0.85, lbm.py:286)            */
+} tlbm_heat;
+
+/* lbm_step with heat enabled (lbm.py:397-513): the flow step of tlbm_step (same arguments and
+ * report behaviour) plus the temperature step from g_in to g_out (7 planes of qstride floats,
+ * distinct buffers).  Needs a handle looked up with spec.heat = 1. */
+int tlbm_step_heat(int handle, const void* f_in, void* f_out, const float* g_in, float* g_out,
+                   const tlbm_params* params, const tlbm_heat* heat, const tlbm_fields* fields,
+                   double* sums, void* stream);
 
 /* ---- derived fields of the output grid (derived.py:48-93) ----------------------------------
  * Grid of mx x my x mz points (x fastest), spacing h, inv_h = 1 / h; its cells are

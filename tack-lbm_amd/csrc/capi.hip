@@ -58,10 +58,12 @@ namespace {
   } while (0)
 
 // ---- specialisation table ------------------------------------------------------------------
-constexpr int kHandles = 3 * 3 * 4 * 3 * 2;  // storage, collision, obstacle, side, diag
+constexpr int kHandles = 2 * 3 * 3 * tlbm::kObstacles * 3 * 2;  // heat, storage, collision,
+                                                                  // obstacle, side, diag
 
 tlbm::StepTable g_tables[3][3];
 tlbm::BatchTable g_batch_tables[3][3];
+tlbm::HeatTable g_heat_tables[3];
 // launch override per handle: threads per block along x (0 = default)
 int g_block_x[kHandles];
 std::once_flag g_once;
@@ -76,11 +78,14 @@ void build_tables() {
   tlbm_register_2_0(g_tables[2][0], g_batch_tables[2][0]);
   tlbm_register_2_1(g_tables[2][1], g_batch_tables[2][1]);
   tlbm_register_2_2(g_tables[2][2], g_batch_tables[2][2]);
+  tlbm_register_heat_0(g_heat_tables[0]);
+  tlbm_register_heat_1(g_heat_tables[1]);
+  tlbm_register_heat_2(g_heat_tables[2]);
   for (int& b : g_block_x) b = 0;
 }
 
 struct Decoded {
-  int storage, collision, obstacle, side, diag;
+  int storage, collision, obstacle, side, diag, heat;
 };
 
 bool decode(int h, Decoded& d) {
@@ -89,12 +94,13 @@ bool decode(int h, Decoded& d) {
   h /= 2;
   d.side = h % 3;
   h /= 3;
-  d.obstacle = h % 4;
-  h /= 4;
+  d.obstacle = h % tlbm::kObstacles;
+  h /= tlbm::kObstacles;
   d.collision = h % 3;
   h /= 3;
-  d.storage = h;
-  return true;
+  d.storage = h % 3;
+  d.heat = h / 3;
+  return !(d.heat && d.storage != TLBM_F32);
 }
 
 // ---- per-device report workspace (grow-only) ----------------------------------------------
@@ -164,6 +170,8 @@ int check_params(const tlbm_params* p, int obstacle = -1) {
   if (!p) return fail(TLBM_ERR_INVALID, "params is NULL");
   if (obstacle == TLBM_FIELD && !p->phi)
     return fail(TLBM_ERR_INVALID, "a TLBM_FIELD obstacle needs params->phi");
+  if (obstacle == TLBM_HEATSINK && (!(p->obs[7] >= 1.0f) || !(p->obs[6] >= 0.0f)))
+    return fail(TLBM_ERR_INVALID, "heat sink fin pitch must be >= 1 and thickness >= 0");
   if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0)
     return fail(TLBM_ERR_INVALID, "grid %dx%dx%d must be positive", p->nx, p->ny, p->nz);
   const uint64_t n = static_cast<uint64_t>(p->nx) * p->ny * p->nz;
@@ -188,7 +196,7 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
   a.tau0 = p.tau0;
   a.smag = 25.456f * p.cs * p.cs;
   a.magic = p.magic;
-  for (int i = 0; i < 6; ++i) a.o[i] = p.obs[i];
+  for (int i = 0; i < 8; ++i) a.o[i] = p.obs[i];
   float r = 0.0f;
   if (obstacle == TLBM_CYLINDER) {
     r = p.obs[2];
@@ -208,7 +216,12 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
   const float ctr[3] = {p.obs[0], p.obs[1], obstacle == TLBM_CYLINDER ? 0.0f : p.obs[2]};
   const int ext[3] = {p.nx, p.ny, p.nz};
   a.wrap = obstacle == TLBM_FIELD ? 7 : 0;  // a field may be solid anywhere
-  for (int d = 0; d < 3 && obstacle != TLBM_FIELD; ++d) {
+  if (obstacle == TLBM_HEATSINK) {
+    // the duct walls (y = 0, ny - 1) are within near()'s bound; the sink box may touch x or z
+    if (p.obs[0] < 2.0f || p.obs[1] > static_cast<float>(p.nx) - 2.0f) a.wrap |= 1;
+    if (p.obs[2] < 2.0f || p.obs[3] > static_cast<float>(p.nz) - 2.0f) a.wrap |= 4;
+  }
+  for (int d = 0; d < 3 && obstacle != TLBM_FIELD && obstacle != TLBM_HEATSINK; ++d) {
     if (d == 2 && obstacle == TLBM_CYLINDER) continue;  // infinite along z: z wraps change nothing
     const float lo = std::fabs(0.0f - ctr[d]), hi = std::fabs(static_cast<float>(ext[d] - 1) - ctr[d]);
     if (lo < a.reach[d] || hi < a.reach[d] || !std::isfinite(lo) || !std::isfinite(hi))
@@ -240,10 +253,13 @@ hipError_t launch_init(const tlbm::StepArgs& a, float seed_amp, float noise_amp,
 
 using InitLaunch = hipError_t (*)(const tlbm::StepArgs&, float, float, uint64_t, dim3, dim3,
                                   hipStream_t);
-const InitLaunch kInit[3][4] = {
-    {&launch_init<0, 0>, &launch_init<1, 0>, &launch_init<2, 0>, &launch_init<3, 0>},
-    {&launch_init<0, 1>, &launch_init<1, 1>, &launch_init<2, 1>, &launch_init<3, 1>},
-    {&launch_init<0, 2>, &launch_init<1, 2>, &launch_init<2, 2>, &launch_init<3, 2>}};
+const InitLaunch kInit[3][tlbm::kObstacles] = {
+    {&launch_init<0, 0>, &launch_init<1, 0>, &launch_init<2, 0>, &launch_init<3, 0>,
+     &launch_init<4, 0>},
+    {&launch_init<0, 1>, &launch_init<1, 1>, &launch_init<2, 1>, &launch_init<3, 1>,
+     &launch_init<4, 1>},
+    {&launch_init<0, 2>, &launch_init<1, 2>, &launch_init<2, 2>, &launch_init<3, 2>,
+     &launch_init<4, 2>}};
 // [storage][obstacle]
 
 // Per-domain totals of a batched report step: block (d, k) sums partials[k][chunk] over the
@@ -502,11 +518,16 @@ int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle) {
   std::call_once(g_once, build_tables);
   const tlbm_spec& s = *spec;
   if (s.collision < 0 || s.collision > 2) return fail(TLBM_ERR_INVALID, "collision %d", s.collision);
-  if (s.obstacle < 0 || s.obstacle > 3) return fail(TLBM_ERR_INVALID, "obstacle %d", s.obstacle);
+  if (s.obstacle < 0 || s.obstacle >= tlbm::kObstacles)
+    return fail(TLBM_ERR_INVALID, "obstacle %d", s.obstacle);
   if (s.diag < 0 || s.diag > 1) return fail(TLBM_ERR_INVALID, "diag %d", s.diag);
   if (s.side < 0 || s.side > 2) return fail(TLBM_ERR_INVALID, "side %d", s.side);
   if (s.storage < 0 || s.storage > 2) return fail(TLBM_ERR_INVALID, "storage %d", s.storage);
-  *handle = ((((s.storage * 3 + s.collision) * 4 + s.obstacle) * 3 + s.side) * 2) + s.diag;
+  if (s.heat < 0 || s.heat > 1) return fail(TLBM_ERR_INVALID, "heat %d", s.heat);
+  if (s.heat && s.storage != TLBM_F32)
+    return fail(TLBM_ERR_INVALID, "the temperature variant needs TLBM_F32 storage");
+  *handle = (((((s.heat * 3 + s.storage) * 3 + s.collision) * tlbm::kObstacles + s.obstacle) * 3 +
+              s.side) * 2) + s.diag;
   return TLBM_OK;
 }
 
@@ -543,17 +564,37 @@ int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, fl
   return TLBM_OK;
 }
 
-int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* params,
+}  // extern "C"
+
+namespace {
+
+// tlbm_step and tlbm_step_heat: `heat` is NULL for the flow step alone.
+int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const float* g_in,
+              float* g_out, const tlbm_params* params, const tlbm_heat* heat,
               const tlbm_fields* fields, double* sums, void* stream) {
-  std::call_once(g_once, build_tables);
-  Decoded d;
-  if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
   if (!f_in || !f_out) return fail(TLBM_ERR_INVALID, "f_in or f_out is NULL");
   if (f_in == f_out) return fail(TLBM_ERR_INVALID, "f_in and f_out must be distinct buffers");
   if (int rc = check_params(params, d.obstacle)) return rc;
   tlbm::StepArgs a = make_args(*params, d.obstacle);
   a.fin = f_in;
   a.fout = f_out;
+  if (heat) {
+    if (!g_in || !g_out) return fail(TLBM_ERR_INVALID, "g_in or g_out is NULL");
+    if (g_in == g_out) return fail(TLBM_ERR_INVALID, "g_in and g_out must be distinct buffers");
+    if (!heat->t_solid) return fail(TLBM_ERR_INVALID, "heat->t_solid is NULL");
+    if (d.diag == TLBM_REPORT && (!heat->temp || !heat->q_solid))
+      return fail(TLBM_ERR_INVALID, "a report step with heat needs heat->temp and heat->q_solid");
+    if (!(heat->alpha >= 0.0f) || !(heat->pr_t > 0.0f))
+      return fail(TLBM_ERR_INVALID, "heat alpha must be >= 0 and pr_t > 0");
+    a.g_in = g_in;
+    a.g_out = g_out;
+    a.t_solid = heat->t_solid;
+    a.q_solid = heat->q_solid;
+    a.temp = heat->temp;
+    a.alpha = heat->alpha;
+    a.nu_mol = heat->nu;
+    a.pr_t = heat->pr_t;
+  }
   const int bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(params->nx, d.storage);
   const uint64_t xblocks = (params->nx + bx - 1) / bx;
   const uint64_t rows = static_cast<uint64_t>(params->ny) * params->nz;
@@ -582,7 +623,9 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
       a.nblocks = nblocks;
     }
   }
-  hipError_t e = g_tables[d.storage][d.collision][d.obstacle][d.side][d.diag](a, grid, block, st);
+  hipError_t e = heat ? g_heat_tables[d.collision][d.obstacle][d.side][d.diag](a, grid, block, st)
+                      : g_tables[d.storage][d.collision][d.obstacle][d.side][d.diag](a, grid, block,
+                                                                                     st);
   if (e != hipSuccess) return hip_fail(e, "lbm_step launch");
   if (d.diag == TLBM_REPORT && sums) {
     hipLaunchKernelGGL(reduce_partials, dim3(TLBM_NSUMS), dim3(1024), 0, st, a.partials, nblocks,
@@ -591,6 +634,30 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
     if (e != hipSuccess) return hip_fail(e, "report reduction launch");
   }
   return TLBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* params,
+              const tlbm_fields* fields, double* sums, void* stream) {
+  std::call_once(g_once, build_tables);
+  Decoded d;
+  if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
+  if (d.heat) return fail(TLBM_ERR_INVALID, "handle %d has heat: step it with tlbm_step_heat", handle);
+  return step_impl(handle, d, f_in, f_out, nullptr, nullptr, params, nullptr, fields, sums, stream);
+}
+
+int tlbm_step_heat(int handle, const void* f_in, void* f_out, const float* g_in, float* g_out,
+                   const tlbm_params* params, const tlbm_heat* heat, const tlbm_fields* fields,
+                   double* sums, void* stream) {
+  std::call_once(g_once, build_tables);
+  Decoded d;
+  if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
+  if (!d.heat) return fail(TLBM_ERR_INVALID, "handle %d has no heat: look it up with heat = 1", handle);
+  if (!heat) return fail(TLBM_ERR_INVALID, "heat is NULL");
+  return step_impl(handle, d, f_in, f_out, g_in, g_out, params, heat, fields, sums, stream);
 }
 
 int tlbm_copy(void* dst, const void* src, size_t bytes, void* stream) {
@@ -634,6 +701,7 @@ int tlbm_batch_create(int handle, const tlbm_params* params, int ndomains, tlbm_
   *out = nullptr;
   Decoded d;
   if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
+  if (d.heat) return fail(TLBM_ERR_INVALID, "batched lattices have no temperature variant");
   if (!params || ndomains <= 0 || ndomains > 65535)
     return fail(TLBM_ERR_INVALID, "need 1..65535 domains (got %d)", ndomains);
   for (int i = 0; i < ndomains; ++i) {

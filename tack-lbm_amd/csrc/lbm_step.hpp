@@ -50,6 +50,14 @@ constexpr float kW[kQ] = {1.0f / 3, 1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18, 
                           1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36,
                           1.0f / 36};
 
+// D3Q7 of the temperature (lbm.py:20-25): rest plus the six faces, c_s^2 = 1/4
+constexpr int kQ7 = 7;
+constexpr int kCX7[kQ7] = {0, 1, -1, 0, 0, 0, 0};
+constexpr int kCY7[kQ7] = {0, 0, 0, 1, -1, 0, 0};
+constexpr int kCZ7[kQ7] = {0, 0, 0, 0, 0, 1, -1};
+constexpr float kW7[kQ7] = {0.25f, 0.125f, 0.125f, 0.125f, 0.125f, 0.125f, 0.125f};
+constexpr int opp7(int q) { return q == 0 ? 0 : (q % 2 == 1 ? q + 1 : q - 1); }
+
 // lbm.py:28-30 and 33-35, evaluated at compile time.
 constexpr int opp(int q) {
   for (int p = 0; p < kQ; ++p)
@@ -111,7 +119,7 @@ struct StepArgs {
   int32_t wrap;              // bit 0/1/2: the obstacle reaches across the x/y/z periodic faces
   float omega;               // BGK
   float tau0, smag, magic;   // smag = 25.456 cs^2 (lbm.py:93)
-  float o[6];                // obstacle parameters
+  float o[8];                // obstacle parameters
   float r2;                  // r*r (cylinder, sphere)
   float reach[3];            // half extents + 2 (cells) for the neighbourhood bound
   float u0, s0, s_inv, s_k;  // sponge: start, 1 / (nx - start), strength
@@ -122,6 +130,13 @@ struct StepArgs {
   double* partials;          // report: per-block FP64 partial sums [TLBM_NSUMS][nblocks]
   uint32_t nblocks;
   uint32_t xblocks, rows;    // launch shape: blocks per x-row, ny * nz rows
+  // temperature variant (lbm.py:457-492)
+  const float* g_in;
+  float* g_out;
+  const float* t_solid;
+  float* q_solid;
+  float* temp;
+  float alpha, nu_mol, pr_t;
 };
 
 // ---- obstacles: lbm.py:144-227 ---------------------------------------------------------------
@@ -140,6 +155,9 @@ struct Cylinder {
     return fabsf(static_cast<float>(x) - a.o[0]) < a.reach[0] &&
            fabsf(static_cast<float>(y) - a.o[1]) < a.reach[1];
   }
+  __device__ static __forceinline__ float sink(const StepArgs& a, int x, int y, int z) {
+    return solid(a, x, y, z);
+  }
   // unit tangent (lbm.py:159-169)
   __device__ static __forceinline__ void wall_vel(const StepArgs& a, float xm, float ym, float,
                                                   float& u, float& v, float& w) {
@@ -157,6 +175,9 @@ struct Sphere {
     const float dy = static_cast<float>(y) - a.o[1];
     const float dz = static_cast<float>(z) - a.o[2];
     return (dx * dx + dy * dy + dz * dz < a.r2) ? 1.0f : 0.0f;
+  }
+  __device__ static __forceinline__ float sink(const StepArgs& a, int x, int y, int z) {
+    return solid(a, x, y, z);
   }
   __device__ static __forceinline__ bool near(const StepArgs& a, int x, int y, int z) {
     return fabsf(static_cast<float>(x) - a.o[0]) < a.reach[0] &&
@@ -177,6 +198,9 @@ struct Box {
                           fabsf(static_cast<float>(z) - a.o[2]) - a.o[5]);
     return (d < 0.0f) ? 1.0f : 0.0f;
   }
+  __device__ static __forceinline__ float sink(const StepArgs& a, int x, int y, int z) {
+    return solid(a, x, y, z);
+  }
   __device__ static __forceinline__ bool near(const StepArgs& a, int x, int y, int z) {
     return fabsf(static_cast<float>(x) - a.o[0]) < a.reach[0] &&
            fabsf(static_cast<float>(y) - a.o[1]) < a.reach[1] &&
@@ -195,7 +219,39 @@ struct Field {
   __device__ static __forceinline__ float solid(const StepArgs& a, int x, int y, int z) {
     return a.phi[(static_cast<uint32_t>(z) * a.ny + y) * static_cast<uint32_t>(a.nx) + x];
   }
+  __device__ static __forceinline__ float sink(const StepArgs& a, int x, int y, int z) {
+    return solid(a, x, y, z);
+  }
   __device__ static __forceinline__ bool near(const StepArgs&, int, int, int) { return true; }
+  __device__ static __forceinline__ void wall_vel(const StepArgs&, float, float, float, float& u,
+                                                  float& v, float& w) {
+    u = v = w = 0.0f;
+  }
+};
+
+// lbm.py:231-268: a finned heat sink on the floor of a duct.  o = {x0, x1, z0, z1, y_base,
+// y_top, fin_thickness, fin_pitch}.  The sink (base plate and fins along x) is solid, and so are
+// the duct's floor and ceiling rows y = 0 and y = ny - 1 (adiabatic walls).  Resting walls.
+struct HeatSink {
+  static constexpr bool kMoving = false;
+  __device__ static __forceinline__ float sink(const StepArgs& a, int x, int y, int z) {
+    const float fx = static_cast<float>(x), fy = static_cast<float>(y), fz = static_cast<float>(z);
+    const bool inside = fx >= a.o[0] && fx < a.o[1] && fz >= a.o[2] && fz < a.o[3] && fy >= 1.0f;
+    const bool base = fy < a.o[4];
+    // z >= z0 wherever `inside` holds, so the remainder is that of a non-negative number
+    const bool fin = fy < a.o[5] && (z - static_cast<int>(a.o[2])) % static_cast<int>(a.o[7]) <
+                                        static_cast<int>(a.o[6]);
+    return inside && (base || fin) ? 1.0f : 0.0f;
+  }
+  __device__ static __forceinline__ float solid(const StepArgs& a, int x, int y, int z) {
+    return (y <= 0 || y >= a.ny - 1) ? 1.0f : sink(a, x, y, z);
+  }
+  __device__ static __forceinline__ bool near(const StepArgs& a, int x, int y, int z) {
+    const float fx = static_cast<float>(x), fy = static_cast<float>(y), fz = static_cast<float>(z);
+    return y <= 1 || y >= a.ny - 2 ||
+           (fx >= a.o[0] - 1.0f && fx < a.o[1] + 1.0f && fz >= a.o[2] - 1.0f &&
+            fz < a.o[3] + 1.0f && fy < a.o[5] + 1.0f);
+  }
   __device__ static __forceinline__ void wall_vel(const StepArgs&, float, float, float, float& u,
                                                   float& v, float& w) {
     u = v = w = 0.0f;
@@ -205,7 +261,10 @@ struct Field {
 template <int OBS>
 using ObstacleT = std::conditional_t<
     OBS == TLBM_CYLINDER, Cylinder,
-    std::conditional_t<OBS == TLBM_SPHERE, Sphere, std::conditional_t<OBS == TLBM_BOX, Box, Field>>>;
+    std::conditional_t<
+        OBS == TLBM_SPHERE, Sphere,
+        std::conditional_t<OBS == TLBM_BOX, Box,
+                           std::conditional_t<OBS == TLBM_FIELD, Field, HeatSink>>>>;
 
 template <int STORAGE>
 using StorageT = std::conditional_t<STORAGE == TLBM_F32, StoreF32,
@@ -294,10 +353,18 @@ __device__ __forceinline__ void block_sums(double (&v)[TLBM_NSUMS], double* part
 
 // ---- one cell's collision and sponge (lbm.py:433-456) ------------------------------------------
 // v: the 19 streamed working values in, the values to store out.
-template <int COLL, bool SHIFT>
+// What the temperature step reads from the flow step (lbm.py:445, 447-451).
+struct CollideExtra {
+  float tau;                // the relaxation time the collision returned
+  float sigma;              // sponge blend
+  float uxs, uys, uzs;      // sponge target velocity
+};
+
+template <int COLL, bool SHIFT, bool EXTRA = false>
 __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], float x,
                                              float here, float& rho_out, float& ux_out,
-                                             float& uy_out, float& uz_out) {
+                                             float& uy_out, float& uz_out,
+                                             CollideExtra* ex = nullptr) {
   // moments (lbm.py:433-444); shifted storage: rho = 1 + sum d
   float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
   static_for<0, kQ>([&](auto qc) {
@@ -329,12 +396,15 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
   if constexpr (COLL == TLBM_BGK) {
     const float k = 1.0f - a.omega;
     static_for<0, kQ>([&](auto qc) { d[decltype(qc)::value] *= k; });
+    if constexpr (EXTRA) ex->tau = 1.0f / a.omega;
   } else if constexpr (COLL == TLBM_LES) {
     const float tau = smagorinsky_tau(d, inv_rho, a.tau0, a.smag);
     const float k = 1.0f - rcp(tau);
     static_for<0, kQ>([&](auto qc) { d[decltype(qc)::value] *= k; });
+    if constexpr (EXTRA) ex->tau = tau;
   } else {
     const float tau = smagorinsky_tau(d, inv_rho, a.tau0, a.smag);
+    if constexpr (EXTRA) ex->tau = tau;
     const float kp = 1.0f - rcp(tau);
     const float km = 1.0f - rcp(a.magic * rcp(tau - 0.5f) + 0.5f);
     d[0] *= kp;
@@ -362,12 +432,23 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
       constexpr int q = decltype(qc)::value;
       v[q] = es[q] + keep * d[q];
     });
+    if constexpr (EXTRA) {
+      ex->uxs = ux_s;
+      ex->uys = uy_s;
+      ex->uzs = uz_s;
+    }
   } else {
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       v[q] = e[q] + d[q];
     });
+    if constexpr (EXTRA) {
+      ex->uxs = ux;
+      ex->uys = uy;
+      ex->uzs = uz;
+    }
   }
+  if constexpr (EXTRA) ex->sigma = sigma;
   rho_out = rho;
   ux_out = ux;
   uy_out = uy;
@@ -404,7 +485,85 @@ __device__ __forceinline__ void tile_of_block(uint32_t b, uint32_t xblocks, uint
 // are block-uniform (scalar registers, uniform branches); in the batched kernel they are per
 // lane.  REPORT: writes the 13 fields at `fidx` (when F.ux is not null) and returns the 14 FP64
 // summands in acc (zero for inactive lanes).
-template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE>
+// ---- the temperature step of one cell (lbm.py:457-492) --------------------------------------------
+// D3Q7 pull, periodic in x, y and z; a link from a solid voxel bounces back, and from a sink voxel
+// anti-bounces to its temperature: g = pulled + wall (b (1 - 2 hot) + 2 hot w t_solid - pulled).
+// BGK relaxation with omega_g = 1 / (4 alpha_eff + 1/2) and the flow's sponge blend.  REPORT:
+// writes the cell temperature and adds fluid * temp to the q_solid of each sink voxel it faces.
+template <int OBS, bool REPORT>
+__device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z, uint32_t idx,
+                                          float here, float ux, float uy, float uz,
+                                          const CollideExtra& ex, bool active) {
+  using Obs = ObstacleT<OBS>;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const uint32_t snx = static_cast<uint32_t>(nx);
+  const uint32_t plane = snx * static_cast<uint32_t>(ny);
+  const int xs[3] = {x == nx - 1 ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};
+  const int ys[3] = {y == ny - 1 ? 0 : y + 1, y, y == 0 ? ny - 1 : y - 1};
+  const int zs[3] = {z == nz - 1 ? 0 : z + 1, z, z == 0 ? nz - 1 : z - 1};
+  const float* __restrict__ gin = a.g_in;
+  float g[kQ7];
+  uint32_t nidx[kQ7];
+  static_for<0, kQ7>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int cx = kCX7[q], cy = kCY7[q], cz = kCZ7[q];
+    nidx[q] = static_cast<uint32_t>(zs[1 + cz]) * plane + static_cast<uint32_t>(ys[1 + cy]) * snx +
+              static_cast<uint32_t>(xs[1 + cx]);
+    g[q] = gin[static_cast<uint64_t>(q) * a.ns + nidx[q]];
+  });
+  const bool wrap = ((a.wrap & 1) && (x == 0 || x == nx - 1)) ||
+                    ((a.wrap & 2) && (y == 0 || y == ny - 1)) ||
+                    ((a.wrap & 4) && (z == 0 || z == nz - 1));
+  const bool walls = wrap || Obs::near(a, x, y, z);
+  if (walls) {
+    static_for<0, kQ7>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = kCX7[q], cy = kCY7[q], cz = kCZ7[q];
+      const float wall = q == 0 ? here : Obs::solid(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]);
+      if (wall != 0.0f) {
+        const float hot = Obs::sink(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]);
+        const float b = gin[static_cast<uint64_t>(opp7(q)) * a.ns + idx];
+        float at_wall = b * (1.0f - 2.0f * hot);
+        if (hot != 0.0f) at_wall += hot * 2.0f * kW7[q] * a.t_solid[nidx[q]];
+        g[q] = g[q] + wall * (at_wall - g[q]);
+      }
+    });
+  }
+  float temp = 0.0f;
+  static_for<0, kQ7>([&](auto qc) { temp += g[decltype(qc)::value]; });
+  const float alpha = a.alpha + fmaxf(0.0f, (ex.tau - 0.5f) / 3.0f - a.nu_mol) / a.pr_t;
+  const float omega_g = 1.0f / (4.0f * alpha + 0.5f);
+  const float keep = 1.0f - ex.sigma;
+  const float temp_s = keep * temp;
+  if (active) {
+    static_for<0, kQ7>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr float cx = kCX7[q], cy = kCY7[q], cz = kCZ7[q];
+      const float eq = kW7[q] * temp * (1.0f + 4.0f * (cx * ux + cy * uy + cz * uz));
+      const float post = g[q] - omega_g * (g[q] - eq);
+      const float eqs = kW7[q] * temp_s * (1.0f + 4.0f * (cx * ex.uxs + cy * ex.uys + cz * ex.uzs));
+      a.g_out[static_cast<uint64_t>(q) * a.ns + idx] = eqs + keep * (post - eq);
+    });
+  }
+  if constexpr (REPORT) {
+    if (active) {
+      const float fluid = 1.0f - here;
+      float tv = fluid * temp;
+      if (here != 0.0f) tv += here * a.t_solid[idx];
+      a.temp[idx] = tv;
+      if (walls && fluid != 0.0f) {
+        static_for<1, kQ7>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          constexpr int cx = kCX7[q], cy = kCY7[q], cz = kCZ7[q];
+          const float hot = Obs::sink(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]);
+          if (hot != 0.0f) atomicAdd(&a.q_solid[nidx[q]], fluid * hot * temp);
+        });
+      }
+    }
+  }
+}
+
+template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool HEAT = false>
 __device__ __forceinline__ void step_cell(const StepArgs& a, const typename StorageT<STORAGE>::T* __restrict__ fin,
                                           typename StorageT<STORAGE>::T* __restrict__ fout,
                                           int x, int y, int z, bool active,
@@ -515,13 +674,15 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
 
   // ---- collision, sponge, store ----
   float rho, ux, uy, uz;
-  collide_cell<COLL, kShift>(a, f, static_cast<float>(x), here, rho, ux, uy, uz);
+  CollideExtra ex;
+  collide_cell<COLL, kShift, HEAT>(a, f, static_cast<float>(x), here, rho, ux, uy, uz, &ex);
   if (active) {
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       fout[static_cast<uint64_t>(q) * a.ns + idx] = St::encode(f[q]);
     });
   }
+  if constexpr (HEAT) heat_cell<OBS, REPORT>(a, x, y, z, idx, here, ux, uy, uz, ex, active);
 
   // ---- report fields and FP64 summands (lbm.py:493-513, 649-661) ----
   if constexpr (REPORT) {
@@ -564,8 +725,10 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
 }
 
 // ---- the step kernel of one lattice -------------------------------------------------------------
-template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE>
+// HEAT: with the temperature step (FP32 populations only).
+template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool HEAT = false>
 __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
+  static_assert(!HEAT || STORAGE == TLBM_F32, "the temperature variant stores FP32");
   using T = typename StorageT<STORAGE>::T;
   uint32_t xb, row;
   tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
@@ -581,9 +744,9 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   const int z = static_cast<int>(row / static_cast<uint32_t>(a.ny));
   const uint32_t idx = (static_cast<uint32_t>(z) * a.ny + y) * static_cast<uint32_t>(a.nx) + x;
   double acc[TLBM_NSUMS];
-  step_cell<COLL, OBS, SIDE, REPORT, STORAGE>(a, static_cast<const T*>(a.fin),
-                                              static_cast<T*>(a.fout), x, y, z, active, a.fld,
-                                              idx, acc);
+  step_cell<COLL, OBS, SIDE, REPORT, STORAGE, HEAT>(a, static_cast<const T*>(a.fin),
+                                                    static_cast<T*>(a.fout), x, y, z, active,
+                                                    a.fld, idx, acc);
   if constexpr (REPORT) {
     if (a.partials) block_sums(acc, a.partials, a.nblocks, blockIdx.x);
   }

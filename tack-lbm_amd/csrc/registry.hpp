@@ -7,18 +7,22 @@
 
 namespace tlbm {
 
+constexpr int kObstacles = 5;  // TLBM_CYLINDER .. TLBM_HEATSINK
+
 using StepLaunch = hipError_t (*)(const StepArgs&, dim3 grid, dim3 block, hipStream_t stream);
 // Launchers of one (storage, collision) unit, indexed [obstacle][side][report].
-using StepTable = StepLaunch[4][3][2];
+using StepTable = StepLaunch[kObstacles][3][2];
+// Launchers of the temperature variant of one collision (FP32), indexed as StepTable.
+using HeatTable = StepLaunch[kObstacles][3][2];
 
-template <int C, int O, int S, bool R, int ST>
+template <int C, int O, int S, bool R, int ST, bool H = false>
 hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t stream) {
-  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST>), grid, block, 0, stream, a);
+  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H>), grid, block, 0, stream, a);
   return hipGetLastError();
 }
 
 using BatchLaunch = hipError_t (*)(const BatchArgs&, dim3 grid, dim3 block, hipStream_t stream);
-using BatchTable = BatchLaunch[4][3][2];
+using BatchTable = BatchLaunch[kObstacles][3][2];
 
 template <int C, int O, int S, bool R, int ST>
 hipError_t launch_batch(const BatchArgs& b, dim3 grid, dim3 block, hipStream_t stream) {
@@ -28,13 +32,24 @@ hipError_t launch_batch(const BatchArgs& b, dim3 grid, dim3 block, hipStream_t s
 
 template <int C, int ST>
 void fill_table(StepTable& t, BatchTable& bt) {
-  static_for<0, 4>([&](auto o) {
+  static_for<0, kObstacles>([&](auto o) {
     static_for<0, 3>([&](auto s) {
       constexpr int O = decltype(o)::value, S = decltype(s)::value;
       t[O][S][0] = &launch_step<C, O, S, false, ST>;
       t[O][S][1] = &launch_step<C, O, S, true, ST>;
       bt[O][S][0] = &launch_batch<C, O, S, false, ST>;
       bt[O][S][1] = &launch_batch<C, O, S, true, ST>;
+    });
+  });
+}
+
+template <int C>
+void fill_heat_table(HeatTable& t) {
+  static_for<0, kObstacles>([&](auto o) {
+    static_for<0, 3>([&](auto s) {
+      constexpr int O = decltype(o)::value, S = decltype(s)::value;
+      t[O][S][0] = &launch_step<C, O, S, false, TLBM_F32, true>;
+      t[O][S][1] = &launch_step<C, O, S, true, TLBM_F32, true>;
     });
   });
 }
@@ -54,3 +69,9 @@ TLBM_DECLARE_UNIT(1, 2)
 TLBM_DECLARE_UNIT(2, 0)
 TLBM_DECLARE_UNIT(2, 1)
 TLBM_DECLARE_UNIT(2, 2)
+
+// One per collision; defined in heat_inst.hip compiled with -DTLBM_COLL=<c>.
+#define TLBM_DECLARE_HEAT_UNIT(c) void tlbm_register_heat_##c(tlbm::HeatTable& t);
+TLBM_DECLARE_HEAT_UNIT(0)
+TLBM_DECLARE_HEAT_UNIT(1)
+TLBM_DECLARE_HEAT_UNIT(2)

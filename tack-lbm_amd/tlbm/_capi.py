@@ -14,7 +14,7 @@ TLBM_OK = 0
 TLBM_NSUMS = 14
 
 BGK, TRT, LES = 0, 1, 2
-CYLINDER, SPHERE, BOX, FIELD = 0, 1, 2, 3
+CYLINDER, SPHERE, BOX, FIELD, HEATSINK = 0, 1, 2, 3, 4
 QUIET, REPORT = 0, 1
 PERIODIC_Y, SLIP_Y, LID_Y = 0, 1, 2
 F32, F16, BF16 = 0, 1, 2
@@ -22,14 +22,15 @@ F32, F16, BF16 = 0, 1, 2
 
 class Spec(ctypes.Structure):
     _fields_ = [("collision", ctypes.c_int32), ("obstacle", ctypes.c_int32),
-                ("diag", ctypes.c_int32), ("side", ctypes.c_int32), ("storage", ctypes.c_int32)]
+                ("diag", ctypes.c_int32), ("side", ctypes.c_int32), ("storage", ctypes.c_int32),
+                ("heat", ctypes.c_int32)]
 
 
 class Params(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
                 ("reserved0", ctypes.c_int32), ("qstride", ctypes.c_int64),
                 ("omega", ctypes.c_float), ("tau0", ctypes.c_float), ("cs", ctypes.c_float),
-                ("magic", ctypes.c_float), ("obs", ctypes.c_float * 6), ("u0", ctypes.c_float),
+                ("magic", ctypes.c_float), ("obs", ctypes.c_float * 8), ("u0", ctypes.c_float),
                 ("sponge_start", ctypes.c_float), ("sponge_strength", ctypes.c_float),
                 ("lid", ctypes.c_float * 3), ("omega_rot", ctypes.c_void_p),
                 ("phi", ctypes.c_void_p)]
@@ -41,6 +42,12 @@ FIELD_NAMES = ("ux", "uy", "uz", "p", "vmag", "r_rho", "r_ux", "r_uy", "r_uz", "
 
 class Fields(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in FIELD_NAMES]
+
+
+class Heat(ctypes.Structure):
+    _fields_ = [("t_solid", ctypes.c_void_p), ("q_solid", ctypes.c_void_p),
+                ("temp", ctypes.c_void_p), ("alpha", ctypes.c_float), ("nu", ctypes.c_float),
+                ("pr_t", ctypes.c_float)]
 
 
 # name -> (restype, argtypes); the exported surface of include/tlbm.h
@@ -87,6 +94,8 @@ SIGNATURES = {
     "tlbm_graph_launch": (_c.c_int, [_vp, _vp]),
     "tlbm_graph_destroy": (_c.c_int, [_vp]),
     "tlbm_device_cus": (_c.c_int, [_c.POINTER(_c.c_int)]),
+    "tlbm_step_heat": (_c.c_int, [_c.c_int, _vp, _vp, _vp, _vp, _c.POINTER(Params),
+                                  _c.POINTER(Heat), _c.POINTER(Fields), _vp, _vp]),
     "tlbm_velocity_gradient": (_c.c_int, [_vp, _vp, _vp, _vp, _c.c_int, _c.c_int, _c.c_int,
                                           _c.c_float, _vp]),
     "tlbm_vortex_fields": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c.c_int, _c.c_int,

@@ -104,6 +104,8 @@ def tune_batch(batch, reps=5, use_cache=True, candidates=None):
 
 def tune_solver(solver, reps=5, use_cache=True, blocks=(64, 128, 256)):
     """Pick threads per block for a Solver's quiet kernel (the report kernel follows it)."""
+    if getattr(solver, "heat_enabled", False):
+        raise NotImplementedError("tune_solver times the flow step; this solver has heat")
     L = lib()
     lat = solver.lat
     key = (f"solver|{device_key()}|{solver.collision_name}|{type(solver.geo).__name__}|"

@@ -19,7 +19,11 @@ Differences, by design:
   * population storage can be FP16 or BF16 deviations f_q - w_q (``storage="f16"/"bf16"``);
   * ``LidY`` adds a moving lid; obstacle centre, sponge start, Cs/magic and a seeded velocity
     noise can be overridden;
-  * the D3Q7 heat variant (lbm.py:279-319, 457-492) is not built: ``heat=True`` raises.
+  * with ``heat=True`` the temperature field follows the report fields: ``heat_fields["temp"]``
+    is device memory (its managed copy is ``output["temp"]``), ``q_solid`` is device memory
+    (the kernel adds to it atomically; read it with ``q_solid()``), ``t_solid`` stays a managed
+    array written from the host as in the reference; the temperature variant needs FP32
+    populations.
 """
 
 from __future__ import annotations
@@ -312,9 +316,53 @@ class SolidField:
         return (0.0,) * 6
 
 
+class HeatSink:
+    """lbm.py:231-268: a finned heat sink on the floor of a duct (sizes in cells).  The base
+    plate and the fins (along x, ``fin_thickness`` cells every ``fin_pitch`` along z) are the
+    sink; the duct's floor and ceiling rows are adiabatic walls."""
+    code = _capi.HEATSINK
+
+    def __init__(self, ny, x0, length, z0, width, base, fin_height, fin_thickness, fin_pitch):
+        if int(fin_pitch) < 1 or int(fin_thickness) < 0:
+            raise ValueError("fin_pitch must be >= 1 and fin_thickness >= 0")
+        self.ny = ny
+        self.x0, self.x1 = float(x0), float(x0 + length)
+        self.z0, self.z1 = float(z0), float(z0 + width)
+        self.y_base = 1.0 + base
+        self.y_top = 1.0 + base + fin_height
+        self.fin_thickness = int(fin_thickness)
+        self.fin_pitch = int(fin_pitch)
+
+    def obs(self):
+        return (self.x0, self.x1, self.z0, self.z1, self.y_base, self.y_top,
+                float(self.fin_thickness), float(self.fin_pitch))
+
+
 class NoHeat:
     """lbm.py:282-308: the thermal variant compiled out (``enabled = 0``)."""
     enabled = 0
+    Q = 7
+    pr_t = 0.85
+
+    def __init__(self, arrays=None):
+        arrays = arrays or {}
+        self.t_solid = arrays.get("t_solid")
+        self.q_solid = arrays.get("q_solid")
+        self.temp = arrays.get("temp")
+        self.alpha = 0.0
+        self.nu = 0.0
+
+
+class Heat(NoHeat):
+    """lbm.py:311-319: the D3Q7 temperature variant, diffusivity ``alpha`` (nu / Pr) plus the
+    subgrid part max(0, (tau - 1/2) / 3 - nu) / pr_t.  ``arrays``: "t_solid", "q_solid", "temp"
+    (n floats each, device or managed)."""
+    enabled = 1
+
+    def __init__(self, arrays, alpha, nu):
+        super().__init__(arrays)
+        self.alpha = alpha
+        self.nu = nu
 
 
 class Quiet:
@@ -367,23 +415,24 @@ class Flow:
 _HANDLES = {}
 
 
-def kernel_handle(collision, obstacle, diag, side, storage):
+def kernel_handle(collision, obstacle, diag, side, storage, heat=0):
     """The specialisation for this combination of plugin types (cached like Tack's cache)."""
-    key = (collision, obstacle, diag, side, storage)
+    key = (collision, obstacle, diag, side, storage, heat)
     h = _HANDLES.get(key)
     if h is None:
-        spec = _capi.Spec(collision, obstacle, diag, side, storage)
+        spec = _capi.Spec(collision, obstacle, diag, side, storage, heat)
         out = ctypes.c_int(-1)
         check(lib().tlbm_kernel_lookup(ctypes.byref(spec), ctypes.byref(out)), "kernel lookup")
         h = _HANDLES[key] = out.value
     return h
 
 
-def set_launch(phys, geo, diag, side, storage, block_x=0):
+def set_launch(phys, geo, diag, side, storage, block_x=0, heat=None):
     """Threads per block along x of one specialisation (process-wide; 0 = the default)."""
     h = kernel_handle(_code(phys, "collision"), _code(geo, "obstacle"),
                       _capi.REPORT if getattr(diag, "enabled", 0) == 1 else _capi.QUIET,
-                      _code(side, "side"), STORAGES[storage])
+                      _code(side, "side"), STORAGES[storage],
+                      1 if getattr(heat, "enabled", 0) == 1 else 0)
     check(lib().tlbm_set_launch(h, int(block_x)), "set launch")
 
 
@@ -396,6 +445,9 @@ def _code(obj, what):
 
 def make_params(lat, phys, geo, flow, qstride=None):
     """The runtime arguments of a step: instance attributes of the plugins."""
+    if isinstance(geo, HeatSink) and int(geo.ny) != lat.ny:
+        raise ValueError(f"HeatSink(ny={geo.ny}) on a lattice with ny = {lat.ny}: its duct walls "
+                         "are the lattice's first and last rows")
     p = _capi.Params()
     p.nx, p.ny, p.nz = lat.nx, lat.ny, lat.nz
     p.qstride = lat.n if qstride is None else int(qstride)
@@ -448,24 +500,37 @@ def lbm_step(lat, phys, geo, flow, diag, f_in, f_out, heat=None, g_in=None, g_ou
 
     ``diag`` is ``Quiet``/``Report``; a ``Report`` step writes its 13 fields and, when ``sums``
     (a ``HostBuffer``/``ManagedArray`` of 14 float64) is given, their FP64 totals plus the total
-    mass.  ``heat`` must be absent or disabled (``enabled == 0``).
+    mass.  With ``heat.enabled == 1`` (a ``Heat``) the temperature populations step from ``g_in``
+    to ``g_out`` too (FP32), and a report step writes ``heat.temp`` and adds to
+    ``heat.q_solid``.
     """
-    if heat is not None and getattr(heat, "enabled", 0) != 0:
-        raise NotImplementedError("the D3Q7 heat variant (lbm.py:457-492) is not built")
     if f_in.storage != f_out.storage:
         raise ValueError(f"f_in is {f_in.storage}, f_out is {f_out.storage}")
     report = getattr(diag, "enabled", 0) == 1
+    hot = heat is not None and getattr(heat, "enabled", 0) == 1
     h = kernel_handle(_code(phys, "collision"), _code(geo, "obstacle"),
                       _capi.REPORT if report else _capi.QUIET, _code(flow.side, "side"),
-                      STORAGES[f_in.storage])
+                      STORAGES[f_in.storage], 1 if hot else 0)
     qs = _qstride(lat, f_in)
     if _qstride(lat, f_out) != qs:
         raise ValueError("f_in and f_out have different plane strides")
     p = make_params(lat, phys, geo, flow, qs)
     fs = _fields_struct(diag.fields) if report else None
-    check(lib().tlbm_step(h, f_in.ptr, f_out.ptr, ctypes.byref(p),
-                          ctypes.byref(fs) if fs is not None else None,
-                          _ptr(sums) if report else None, stream), "lbm_step")
+    if not hot:
+        check(lib().tlbm_step(h, f_in.ptr, f_out.ptr, ctypes.byref(p),
+                              ctypes.byref(fs) if fs is not None else None,
+                              _ptr(sums) if report else None, stream), "lbm_step")
+        return
+    if g_in is None or g_out is None:
+        raise ValueError("the temperature variant needs g_in and g_out")
+    for g in (g_in, g_out):
+        if g.storage != "f32" or g.size < 7 * qs:
+            raise ValueError(f"g buffers hold 7 FP32 planes of {qs} (got {g.size} {g.storage})")
+    ht = _capi.Heat(_ptr(heat.t_solid), _ptr(heat.q_solid), _ptr(heat.temp), heat.alpha,
+                    heat.nu, heat.pr_t)
+    check(lib().tlbm_step_heat(h, f_in.ptr, f_out.ptr, g_in.ptr, g_out.ptr, ctypes.byref(p),
+                               ctypes.byref(ht), ctypes.byref(fs) if fs is not None else None,
+                               _ptr(sums) if report else None, stream), "lbm_step (heat)")
 
 
 def coarsen(lat, s, src, dst, mx, my, mz, stream=None):
@@ -502,8 +567,8 @@ class Solver:
                  lid_u=(0.04, 0.0, 0.0), center=None, sponge_start=None, sponge_strength=1.0,
                  cs=None, magic=None, seed_amp=0.05, noise_amp=0.0, noise_seed=0,
                  qstride=None, stream=None, field_memory="device"):
-        if heat:
-            raise NotImplementedError("the D3Q7 heat variant (lbm.py:279-319) is not built")
+        if heat and storage != "f32":
+            raise ValueError("the temperature variant needs FP32 populations (storage='f32')")
         _select_device(device)
         self.stream = stream
         self.lateral = lateral
@@ -539,38 +604,70 @@ class Solver:
         self.quiet = Quiet(self.fields)
         self.report = Report(self.fields)
         self.sums = HostBuffer(_capi.TLBM_NSUMS)
-        self.heat_enabled = False
-        self.heat = NoHeat()
+        self.heat_enabled = bool(heat)
         self.prandtl = prandtl
+        self._setup_heat(n)
         self.stride = stride
         self.out_dims = tuple((d + stride - 1) // stride for d in (nx, ny, nz))
+        outputs = self.OUTPUT + (("temp",) if self.heat_enabled else ())
         if stride > 1 or field_memory == "device":
             m = self.out_dims[0] * self.out_dims[1] * self.out_dims[2]
-            self.output = {name: ManagedArray(m) for name in self.OUTPUT}
+            self.output = {name: ManagedArray(m) for name in outputs}
         else:
-            self.output = {name: self.fields[name] for name in self.OUTPUT}
+            self.output = {name: self.heat_fields.get(name) or self.fields[name]
+                           for name in outputs}
         lbm_init(self.lat, self.geo, self.flow, self.f[0], seed_amp, noise_amp, noise_seed,
                  self.stream)
         self.step(report=True)
         self.step_count = 0
 
+    def _setup_heat(self, n):
+        """lbm.py:586-600: the temperature fields (n floats each when heat is on, else one)
+        and the D3Q7 population buffers, zero (lbm.py:594)."""
+        small = n if self.heat_enabled else 1
+        temp = zeros((small,)) if self.field_memory == "device" else ManagedArray(small)
+        self.heat_fields = {"t_solid": ManagedArray(small), "q_solid": zeros((small,)),
+                            "temp": temp}
+        self.g = [zeros((7 * small,)), zeros((7 * small,))]
+        self._make_heat()
+
+    def _make_heat(self):
+        if self.heat_enabled:
+            self.heat = Heat(self.heat_fields, self.nu / self.prandtl, self.nu)
+        else:
+            self.heat = NoHeat(self.heat_fields)
+
+    def set_wall_temperature(self, value):
+        """lbm.py:602-604: fill every solid voxel's temperature (t_solid is a managed array;
+        set parts of it through ``heat_fields["t_solid"].numpy``)."""
+        self.heat_fields["t_solid"].numpy[:] = value
+
+    def q_solid(self):
+        """A host copy of q_solid: per sink voxel, the sum of the temperatures of the fluid cells
+        facing it, accumulated by the last report step (lbm.py:484-492)."""
+        sync(self.stream)
+        return self.heat_fields["q_solid"].numpy()
+
     def update_output(self):
         """lbm.py:606-611: refresh the host-visible output arrays (coarsened when stride > 1)."""
+        def source(name):
+            return self.heat_fields[name] if name == "temp" else self.fields[name]
         if self.stride > 1:
             mx, my, mz = self.out_dims
             for name in self.output:
-                coarsen(self.lat, self.stride, self.fields[name], self.output[name], mx, my, mz,
+                coarsen(self.lat, self.stride, source(name), self.output[name], mx, my, mz,
                         self.stream)
         elif self.field_memory == "device":
             for name in self.output:
-                src = self.fields[name]
-                check(lib().tlbm_memcpy_async(self.output[name].ptr, src.ptr, src.nbytes,
-                                              self.stream), "output copy")
+                src = source(name)
+                check(lib().tlbm_memcpy_async(self.output[name].ptr, src.ptr,
+                                              4 * self.lat.n, self.stream), "output copy")
 
     def set_launch(self, block_x=0):
         """Threads per block of this solver's quiet and report kernels (see ``set_launch``)."""
         for diag in (self.quiet, self.report):
-            set_launch(self.phys, self.geo, diag, self.flow.side, self.storage, block_x)
+            set_launch(self.phys, self.geo, diag, self.flow.side, self.storage, block_x,
+                       self.heat)
 
     def set_physics(self, collision, reynolds):
         """lbm.py:613-616: nu = u0 D / Re; a collision change selects another kernel."""
@@ -582,6 +679,10 @@ class Solver:
         if collision == "trt" and self._magic is not None:
             kw["magic"] = self._magic
         self.phys = COLLISIONS[collision](self.nu, **kw)
+        if getattr(self, "heat_enabled", False):
+            # the reference means to rebuild Heat here (lbm.py:625-626, unreachable after the
+            # return of omega_rot); the diffusivity follows the new viscosity
+            self._make_heat()
 
     def set_actuation(self, omega_rot):
         """lbm.py:618-620: cylinder surface speed in lattice units."""
@@ -592,7 +693,7 @@ class Solver:
         return float(self._omega_state.numpy[0])
 
     def set_obstacle(self, kind):
-        """lbm.py:628-637 (``heatsink`` is part of the heat variant and not built)."""
+        """lbm.py:628-637."""
         nx, ny, nz, r = self.nx, self.ny, self.nz, self.diameter / 2
         if self._center is None:
             x0, y0, z0 = 0.22 * nx, 0.5 * ny + 1.7, 0.5 * nz
@@ -601,7 +702,11 @@ class Solver:
         self.obstacle_name = kind
         makers = {"cylinder": lambda: Cylinder(x0, y0, r),
                   "sphere": lambda: Sphere(x0, y0, z0, r),
-                  "box": lambda: Box(x0, y0, z0, r, r, 0.3 * nz)}
+                  "box": lambda: Box(x0, y0, z0, r, r, 0.3 * nz),
+                  "heatsink": lambda: HeatSink(ny, int(0.2 * nx), int(0.3 * nx), int(0.125 * nz),
+                                               int(0.75 * nz), round(self.diameter / 2),
+                                               int(0.45 * ny), round(self.diameter / 3),
+                                               round(self.diameter))}
         if kind not in makers:
             raise KeyError(f"obstacle {kind!r} is not built (have {sorted(makers)})")
         self.geo = makers[kind]()
@@ -610,14 +715,19 @@ class Solver:
         """The kernel handle of this solver's quiet or report step."""
         return kernel_handle(_code(self.phys, "collision"), _code(self.geo, "obstacle"),
                              _capi.REPORT if report else _capi.QUIET,
-                             _code(self.flow.side, "side"), STORAGES[self.storage])
+                             _code(self.flow.side, "side"), STORAGES[self.storage],
+                             1 if self.heat_enabled else 0)
 
     def step(self, report=False):
         """lbm.py:639-647: one step, then swap the ping-pong buffers."""
         diag = self.report if report else self.quiet
+        if report and self.heat_enabled:
+            self.heat_fields["q_solid"].zero_(self.stream)
         lbm_step(self.lat, self.phys, self.geo, self.flow, diag, self.f[0], self.f[1],
-                 sums=self.sums if report else None, stream=self.stream)
+                 self.heat, self.g[0], self.g[1], sums=self.sums if report else None,
+                 stream=self.stream)
         self.f.reverse()
+        self.g.reverse()
         self.step_count += 1
 
     def monitors(self, interval):

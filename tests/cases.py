@@ -12,12 +12,16 @@ GEOMETRIES = {
     "cylinder": (9.3, 6.2, 3.1),
     "sphere": (9.3, 6.2, 4.4, 3.1),
     "box": (9.3, 6.2, 4.4, 2.6, 2.1, 2.4),
+    # HeatSink(ny, x0, length, z0, width, base, fin_height, fin_thickness, fin_pitch): for a
+    # lattice 12 cells high
+    "heatsink": (12, 4, 9, 1, 7, 2, 4, 1, 3),
 }
 # an obstacle that straddles the periodic x boundary (pull sources wrap into the solid)
 WRAP_GEOMETRIES = {
     "cylinder": (0.4, 5.3, 2.9),
     "sphere": (0.4, 5.3, 3.6, 2.9),
     "box": (0.4, 5.3, 3.6, 2.2, 1.9, 2.1),
+    "heatsink": (12, 0, 30, 0, 20, 1, 3, 2, 4),   # touching the x and z faces
 }
 LID = (0.04, 0.01, -0.02)
 
@@ -73,8 +77,8 @@ class Case:
         if self.obstacle == "field":
             geo = O.Field(self.geometry, self.dims)
         else:
-            geo = {"cylinder": O.Cylinder, "sphere": O.Sphere, "box": O.Box}[self.obstacle](
-                *self.geometry)
+            geo = {"cylinder": O.Cylinder, "sphere": O.Sphere, "box": O.Box,
+                   "heatsink": O.HeatSink}[self.obstacle](*self.geometry)
         side = O.LidY(*LID) if self.side == "lid" else O.SIDES[self.side]()
         flow = O.Flow(self.u0, self.sponge_start, self.sponge_strength, self.omega_rot, side)
         return phys, geo, flow
@@ -88,8 +92,8 @@ class Case:
         if self.obstacle == "field":
             geo = tlbm.SolidField(tlbm.array(self.geometry))
         else:
-            geo = {"cylinder": tlbm.Cylinder, "sphere": tlbm.Sphere, "box": tlbm.Box}[
-                self.obstacle](*self.geometry)
+            geo = {"cylinder": tlbm.Cylinder, "sphere": tlbm.Sphere, "box": tlbm.Box,
+                   "heatsink": tlbm.HeatSink}[self.obstacle](*self.geometry)
         side = tlbm.LidY(*LID) if self.side == "lid" else {"periodic": tlbm.PeriodicY,
                                                           "slip": tlbm.SlipY}[self.side]()
         omega = tlbm.array(np.array([self.omega_rot], np.float32))

@@ -59,29 +59,60 @@ int main(void) {
 
 
 def test_kernel_lookup_covers_every_combination():
+    """3 storages x 3 collisions x 5 obstacles x 3 sides x 2 diagnostics, plus the temperature
+    variant of every FP32 combination: distinct handles."""
     from tlbm import _capi
     lib = _capi.lib()
     seen = set()
-    for st in range(3):
-        for c in range(3):
-            for o in range(4):
-                for s in range(3):
-                    for d in range(2):
-                        h = ctypes.c_int(-1)
-                        spec = _capi.Spec(c, o, d, s, st)
-                        assert lib.tlbm_kernel_lookup(ctypes.byref(spec), ctypes.byref(h)) == 0
-                        seen.add(h.value)
-    assert len(seen) == 216
+    for heat in range(2):
+        for st in range(1 if heat else 3):
+            for c in range(3):
+                for o in range(5):
+                    for s in range(3):
+                        for d in range(2):
+                            h = ctypes.c_int(-1)
+                            spec = _capi.Spec(c, o, d, s, st, heat)
+                            assert lib.tlbm_kernel_lookup(ctypes.byref(spec),
+                                                          ctypes.byref(h)) == 0
+                            seen.add(h.value)
+    assert len(seen) == 270 + 90
 
 
-@pytest.mark.parametrize("bad", [(3, 0, 0, 0, 0), (0, -1, 0, 0, 0), (0, 4, 0, 0, 0), (0, 0, 2, 0, 0),
-                                 (0, 0, 0, 3, 0), (0, 0, 0, 0, 3)])
+@pytest.mark.parametrize("bad", [(3, 0, 0, 0, 0, 0), (0, -1, 0, 0, 0, 0), (0, 5, 0, 0, 0, 0),
+                                 (0, 0, 2, 0, 0, 0), (0, 0, 0, 3, 0, 0), (0, 0, 0, 0, 3, 0),
+                                 (0, 0, 0, 0, 0, 2), (0, 0, 0, 0, 1, 1)])
 def test_kernel_lookup_rejects_unknown_specs(bad):
     from tlbm import _capi
     lib = _capi.lib()
     h = ctypes.c_int(-1)
     assert lib.tlbm_kernel_lookup(ctypes.byref(_capi.Spec(*bad)), ctypes.byref(h)) != 0
     assert _capi.last_error()
+
+
+def test_heat_and_flow_entry_points_reject_each_others_handles():
+    from tlbm import _capi
+    lib = _capi.lib()
+    flow, heat = ctypes.c_int(-1), ctypes.c_int(-1)
+    lib.tlbm_kernel_lookup(ctypes.byref(_capi.Spec(0, 4, 0, 0, 0, 0)), ctypes.byref(flow))
+    lib.tlbm_kernel_lookup(ctypes.byref(_capi.Spec(0, 4, 0, 0, 0, 1)), ctypes.byref(heat))
+    p = _capi.Params()
+    p.nx, p.ny, p.nz, p.qstride = 8, 6, 4, 192
+    p.obs[6], p.obs[7] = 1.0, 3.0
+    a, b, c, d = (ctypes.c_void_p(v) for v in (64, 128, 256, 512))
+    assert lib.tlbm_step(heat, a, b, ctypes.byref(p), None, None, None) != 0
+    assert "tlbm_step_heat" in _capi.last_error()
+    ht = _capi.Heat(None, None, None, 0.01, 0.01, 0.85)
+    assert lib.tlbm_step_heat(flow, a, b, c, d, ctypes.byref(p), ctypes.byref(ht), None, None,
+                              None) != 0
+    assert lib.tlbm_step_heat(heat, a, b, c, d, ctypes.byref(p), ctypes.byref(ht), None, None,
+                              None) != 0                                     # no t_solid
+    assert "t_solid" in _capi.last_error()
+    ht.t_solid = 1024
+    assert lib.tlbm_step_heat(heat, a, b, c, c, ctypes.byref(p), ctypes.byref(ht), None, None,
+                              None) != 0                                     # g_in == g_out
+    p.obs[7] = 0.0
+    assert lib.tlbm_step_heat(heat, a, b, c, d, ctypes.byref(p), ctypes.byref(ht), None, None,
+                              None) != 0                                     # fin pitch 0
 
 
 def test_step_argument_errors_are_reported_before_any_launch():

@@ -78,7 +78,8 @@ def _run_step_inner(case, f64, report, storage, seed, lat, phys, geo, flow):
     return got, ref, gfields, o_fields, sums.numpy.copy()
 
 
-COMBOS = [(c, o, s) for c in ("bgk", "trt", "les") for o in ("cylinder", "sphere", "box", "field")
+COMBOS = [(c, o, s) for c in ("bgk", "trt", "les")
+          for o in ("cylinder", "sphere", "box", "field", "heatsink")
           for s in ("periodic", "slip", "lid")]
 
 
@@ -128,10 +129,11 @@ def test_report_step_f32(collision, obstacle, side, block_x):
     assert abs(sums[13] - float(o["_rho"].sum())) < 2e-7 * case.n
 
 
-@pytest.mark.parametrize("obstacle", ["cylinder", "sphere", "box"])
+@pytest.mark.parametrize("obstacle", ["cylinder", "sphere", "box", "heatsink"])
 @pytest.mark.parametrize("side", ["periodic", "slip"])
 def test_obstacle_across_periodic_x(obstacle, side):
-    case = Case((20, 11, 8), "les", obstacle, side, geometry=WRAP_GEOMETRIES[obstacle])
+    dims = (20, 12, 8) if obstacle == "heatsink" else (20, 11, 8)
+    case = Case(dims, "les", obstacle, side, geometry=WRAP_GEOMETRIES[obstacle])
     f = random_populations(case.n, seed=13)
     got, ref, *_ = _run_step(case, f, report=True)
     assert _rel(got, ref) < 1e-6
