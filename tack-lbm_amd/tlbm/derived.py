@@ -81,6 +81,10 @@ class DerivedFields:
             self._vel = None
         self.jacobian = empty((9 * self.n,)) if gradient == "lattice" else None
         self._out = {k: empty((self.cells,)) for k in set(self.kinds.values())}
+        # the steering status the Catalyst channel reports (derived.py:113-115); no steered
+        # expressions without the expression compiler
+        self.error_column = 0
+        self.message = "ok: no steered fields"
 
     @property
     def names(self):

@@ -1,0 +1,108 @@
+"""The Catalyst channel tree of a Solver (run_lbm.py:38-121): the Conduit paths the reference
+builds, zero-copy vertex fields, SI scaling, the monitor history, with and without heat."""
+
+import numpy as np
+import pytest
+
+
+class _Managed:
+    def __init__(self, n, value):
+        self.numpy = np.full(n, value, np.float32)
+
+
+class _FakeSolver:
+    collision_name, reynolds, u0, diameter, stride = "les", 300.0, 0.08, 24.0, 2
+    out_dims = (4, 3, 2)
+    omega_rot = 0.04
+
+    def __init__(self, heat):
+        names = ("ux", "uy", "uz", "p", "vmag") + (("temp",) if heat else ())
+        self.output = {k: _Managed(24, i) for i, k in enumerate(names)}
+
+
+# run_lbm.py:62-115: every leaf the reference sets for the grid and monitor channels
+GRID = ["catalyst/channels/grid/type",
+        "catalyst/channels/grid/data/coordsets/coords/type",
+        *[f"catalyst/channels/grid/data/coordsets/coords/dims/{a}" for a in "ijk"],
+        *[f"catalyst/channels/grid/data/coordsets/coords/origin/{a}" for a in "xyz"],
+        *[f"catalyst/channels/grid/data/coordsets/coords/spacing/d{a}" for a in "xyz"],
+        "catalyst/channels/grid/data/topologies/mesh/type",
+        "catalyst/channels/grid/data/topologies/mesh/coordset",
+        "catalyst/channels/grid/data/fields/velocity/association",
+        "catalyst/channels/grid/data/fields/velocity/topology",
+        *[f"catalyst/channels/grid/data/fields/velocity/values/{a}" for a in "xyz"],
+        *[f"catalyst/channels/grid/data/fields/{f}/{leaf}" for f in ("pressure", "speed")
+          for leaf in ("association", "topology", "values")]]
+STATE = ["catalyst/state/timestep", "catalyst/state/time", "catalyst/state/parameters"]
+
+
+@pytest.mark.parametrize("heat", [False, True])
+def test_channel_tree_has_the_reference_paths_and_zero_copy_fields(heat):
+    from tlbm.catalyst import MONITOR_NAMES, CatalystChannels, _get, paths
+    s = _FakeSolver(heat)
+    ch = CatalystChannels(s)
+    tree = ch.execute(100, 0.33, {"drag": 1.5, "lift": 0.1})
+    want = STATE + GRID + ["catalyst/channels/monitors/type"] + \
+        [f"catalyst/channels/monitors/data/values/{n}" for n in ("iteration", *MONITOR_NAMES)]
+    if heat:
+        want += [f"catalyst/channels/grid/data/fields/temperature/{leaf}"
+                 for leaf in ("association", "topology", "values")]
+    assert paths(tree) == sorted(want)
+    mesh = "catalyst/channels/grid/data"
+    assert _get(tree, f"{mesh}/fields/velocity/values/x") is s.output["ux"].numpy
+    assert _get(tree, f"{mesh}/fields/speed/values") is s.output["vmag"].numpy
+    assert _get(tree, f"{mesh}/coordsets/coords/spacing/dx") == 2 / 24.0
+    assert [_get(tree, f"{mesh}/coordsets/coords/dims/{a}") for a in "ijk"] == [4, 3, 2]
+    assert _get(tree, "catalyst/state/parameters") == ["collision=les", "reynolds=300.0",
+                                                       "rotation=0.5"]
+    assert _get(tree, f"{mesh}/fields/velocity/association") == "vertex"
+
+
+def test_scales_history_and_steering_values():
+    from tlbm.catalyst import CatalystChannels, _get, steering_monitors
+    s = _FakeSolver(False)
+    ch = CatalystChannels(s, scales={"length": 0.01, "pressure": (2.0, 1.0)}, monitor_history=2)
+    for step in (10, 20, 30):
+        m = steering_monitors(s, {"drag": step / 10.0}, step)
+        tree = ch.execute(step, m["time"], m)
+    mesh = "catalyst/channels/grid/data"
+    assert _get(tree, f"{mesh}/coordsets/coords/spacing/dy") == 2 * 0.01
+    assert np.all(_get(tree, f"{mesh}/fields/pressure/values") == 2.0 * 3 + 1.0)
+    vals = "catalyst/channels/monitors/data/values"
+    assert list(_get(tree, f"{vals}/iteration")) == [20.0, 30.0]
+    assert list(_get(tree, f"{vals}/drag")) == [2.0, 3.0]
+    assert list(_get(tree, f"{vals}/collision")) == [2.0, 2.0]
+    assert abs(_get(tree, f"{vals}/time")[-1] - 30 * 0.08 / 24.0) < 1e-15
+    assert np.isnan(_get(tree, f"{vals}/energy")).all()
+
+
+def test_to_conduit_needs_conduit():
+    from tlbm.catalyst import to_conduit
+    try:
+        import conduit  # noqa: F401
+    except ImportError:
+        with pytest.raises(ImportError):
+            to_conduit({"a": 1})
+
+
+@pytest.mark.gpu
+def test_channels_of_a_running_solver():
+    import tlbm
+    from tlbm.catalyst import CatalystChannels, _get, steering_monitors
+    from tlbm.derived import DerivedFields
+    s = tlbm.Solver(48, 24, 16, collision="les", obstacle="cylinder", reynolds=300.0, stride=2)
+    d = DerivedFields(s)
+    ch = CatalystChannels(s, derived=d)
+    for _ in range(20):
+        s.step()
+    s.step(report=True)
+    m = steering_monitors(s, s.monitors(20), s.step_count, d)
+    tree = ch.execute(s.step_count, m["time"], m)
+    mesh = "catalyst/channels/grid/data"
+    ux = _get(tree, f"{mesh}/fields/velocity/values/x")
+    assert np.shares_memory(ux, s.output["ux"].numpy) and ux.size == 24 * 12 * 8
+    assert np.abs(ux).max() > 0
+    vort = _get(tree, f"{mesh}/fields/vorticity")
+    assert vort["association"] == "element" and vort["values"].size == 23 * 11 * 7
+    assert list(_get(tree, "catalyst/channels/derived_status/data/values/message")) == \
+        list(b"ok: no steered fields")
