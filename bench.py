@@ -18,9 +18,10 @@ a hipGraph), FP32 and FP16 storage, reported under "sweep".
 adjoint: the gradient workload (BGK 128x64x64, 2000 steps checkpointed every 100), forward and
 backward pass times and the adjoint kernel against the forward step kernel, under "adjoint".
 derived: vorticity magnitude and Q-criterion of the 512x256x256 output grid, under "derived".
+heat: the temperature variant's step against the flow step (heat-sink lattice), under "heat".
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
-                       [--no-adjoint] [--no-derived] [--storage f32,f16,bf16] [--block N]
+                       [--no-adjoint] [--no-derived] [--no-heat] [--storage f32,f16,bf16] [--block N]
                        [--calibrate]
 """
 
@@ -306,7 +307,43 @@ def time_derived(reps=20):
             "achieved_gbs": algorithmic / (ms * 1e-3) / 1e9, "algorithmic_bytes": algorithmic}
 
 
-def cpu_baseline(max_seconds=20.0):
+def time_heat(reps=20):
+    """The temperature variant (heat=True, lbm.py:457-492) on the headline lattice with a heat
+    sink in place of the sphere (LES, 512x256x256, the Solver's heat-sink placement), against the
+    flow-only step of the same lattice: quiet-kernel means from HIP events."""
+    import tlbm
+    from tlbm._capi import check, lib
+    L = lib()
+    stream = ctypes.c_void_p()
+    check(L.tlbm_stream_create(ctypes.byref(stream)), "stream create")
+    out = {}
+    for heat in (False, True):
+        s = tlbm.Solver(NX, NY, NZ, collision="les", obstacle="heatsink", reynolds=RE, u0=U0,
+                        diameter=2 * RADIUS, heat=heat, stream=stream.value)
+        if heat:
+            s.set_wall_temperature(1.0)
+        for _ in range(3):
+            s.step()
+        ev = Events(L)
+        e0, e1 = ev.new(), ev.new()
+        tlbm.sync(stream.value)
+        check(L.tlbm_event_record(e0, stream), "record")
+        for _ in range(reps):
+            s.step()
+        check(L.tlbm_event_record(e1, stream), "record")
+        tlbm.sync(stream.value)
+        out["heat" if heat else "flow"] = ev.elapsed(e0, e1) / reps
+        del s
+    L.tlbm_stream_destroy(stream)
+    n = NX * NY * NZ
+    return {"workload": "D3Q19 LES + D3Q7 temperature 512x256x256, heat sink, Re 3000",
+            "flow_step_ms": out["flow"], "heat_step_ms": out["heat"],
+            "heat_over_flow": out["heat"] / out["flow"],
+            "achieved_gbs": (152 + 56) * n / (out["heat"] * 1e-3) / 1e9,
+            "algorithmic_bytes_per_cell": 152 + 56}
+
+
+def cpu_baseline(max_seconds=15.0):
     """The NumPy float32 restatement on a 512x256x4 slab of the workload, one core."""
     import numpy as np
     from oracle import lbm_oracle as O
@@ -319,7 +356,7 @@ def cpu_baseline(max_seconds=20.0):
     while True:
         s.step()
         steps += 1
-        if time.perf_counter() - t0 > max_seconds or steps >= 20:
+        if time.perf_counter() - t0 > max_seconds or steps >= 100:
             break
     dt = time.perf_counter() - t0
     return {"value": s.n * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
@@ -340,6 +377,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched parameter sweep")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--no-derived", action="store_true", help="skip the derived-field timing")
+    ap.add_argument("--no-heat", action="store_true", help="skip the temperature-variant timing")
     ap.add_argument("--calibrate", action="store_true",
                     help="also time a plain 1 GiB copy (counter calibration, streaming ceiling)")
     args = ap.parse_args()
@@ -373,6 +411,12 @@ def main():
         derived = time_derived()
         print(f"[bench] derived fields: {derived['ms']:.3f} ms ({derived['achieved_gbs']:.0f} "
               f"GB/s) vs one step {derived['step_ms']:.3f} ms", file=sys.stderr, flush=True)
+    heat = None
+    if not args.no_heat:
+        heat = time_heat()
+        print(f"[bench] heat: step {heat['heat_step_ms']:.3f} ms vs flow {heat['flow_step_ms']:.3f} "
+              f"ms ({heat['heat_over_flow']:.2f}x, {heat['achieved_gbs']:.0f} GB/s)",
+              file=sys.stderr, flush=True)
     copy = calibrate_copy() if args.calibrate else None
     if copy:
         print(f"[bench] copy kernel: {copy['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -414,6 +458,8 @@ def main():
     if adjoint:
         line["adjoint"] = {k: (round(v, 6) if isinstance(v, float) else v)
                            for k, v in adjoint.items()}
+    if heat:
+        line["heat"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in heat.items()}
     if derived:
         line["derived"] = {k: (round(v, 4) if isinstance(v, float) else v)
                            for k, v in derived.items()}

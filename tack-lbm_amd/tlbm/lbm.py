@@ -640,7 +640,9 @@ class Solver:
         temp = zeros((small,)) if self.field_memory == "device" else ManagedArray(small)
         self.heat_fields = {"t_solid": ManagedArray(small), "q_solid": zeros((small,)),
                             "temp": temp}
-        self.g = [zeros((7 * small,)), zeros((7 * small,))]
+        # the temperature planes share the populations' plane stride
+        planes = self.qstride if self.heat_enabled else 1
+        self.g = [zeros((7 * planes,)), zeros((7 * planes,))]
         self._make_heat()
 
     def _make_heat(self):

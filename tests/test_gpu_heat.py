@@ -139,3 +139,23 @@ def test_solver_heat_wiring():
     assert solid.any() and np.all(temp[solid] == 1.0)
     with pytest.raises(ValueError):
         tlbm.Solver(16, 12, 8, obstacle="heatsink", heat=True, storage="f16")
+
+
+def test_solver_heat_with_padded_planes():
+    """The temperature planes follow the populations' plane stride: a padded qstride gives the
+    same state, bit for bit, as the unpadded one."""
+    import tlbm
+    out = []
+    for qstride in (None, 48 * 24 * 16 + 320):
+        s = tlbm.Solver(48, 24, 16, collision="les", obstacle="heatsink", reynolds=500.0,
+                        heat=True, diameter=6.0, qstride=qstride)
+        s.set_wall_temperature(1.0)
+        for _ in range(9):
+            s.step()
+        s.step(report=True)
+        s.update_output()
+        tlbm.sync()
+        g = s.g[0].numpy().reshape(7, s.qstride)[:, :s.lat.n]
+        out.append((s.populations(), g, s.output["temp"].numpy.copy(), s.q_solid()))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
