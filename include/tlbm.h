@@ -217,8 +217,9 @@ int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* 
  * No counterpart in the reference, which steps one lattice per launch.  A batch is a set of
  * independent lattices that share one specialisation (collision, obstacle, side, storage of
  * `handle`; its diag is ignored, both variants are available) and differ in size and runtime
- * arguments (params[d]; qstride is ignored).  Their populations live in one buffer: domain d at
- * element offset elem_offset(d), q-major with plane stride nx*ny*nz of that domain; packed
+ * arguments (params[d]).  Their populations live in one buffer: domain d at element offset
+ * elem_offset(d), q-major with plane stride params[d].qstride (a value below the domain's
+ * nx*ny*nz means nx*ny*nz: packed), the domains one after another; packed
  * per-cell arrays (report fields) hold domain d at cell_offset(d).  One launch steps every
  * domain: a persistent grid walks chunks of block_x consecutive cells of one domain. */
 typedef struct tlbm_batch tlbm_batch;

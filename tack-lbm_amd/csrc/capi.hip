@@ -331,15 +331,14 @@ int batch_build(tlbm_batch* b) {
     const tlbm_params& p = b->params[d];
     const uint64_t n = static_cast<uint64_t>(p.nx) * p.ny * p.nz;
     tlbm::BatchDomain& D = b->dom[d];
-    D.a = make_args(p, b->spec.obstacle);
-    D.a.ns = n;
+    D.a = make_args(p, b->spec.obstacle);  // a.ns = the domain's plane stride (>= n)
     D.elem_off = elem;
     D.cell_off = static_cast<uint32_t>(cell);
     D.chunk_off = static_cast<uint32_t>(chunk);
     D.inv_nx = 1.0f / static_cast<float>(p.nx);
     D.inv_nxny = 1.0f / static_cast<float>(static_cast<uint64_t>(p.nx) * p.ny);
     b->chunk_off[d] = static_cast<uint32_t>(chunk);
-    elem += 19 * n;
+    elem += 19 * static_cast<uint64_t>(p.qstride);
     cell += n;
     chunk += (n + b->block - 1) / b->block;
   }
@@ -706,18 +705,20 @@ int tlbm_batch_create(int handle, const tlbm_params* params, int ndomains, tlbm_
     return fail(TLBM_ERR_INVALID, "need 1..65535 domains (got %d)", ndomains);
   for (int i = 0; i < ndomains; ++i) {
     tlbm_params p = params[i];
-    p.qstride = static_cast<int64_t>(p.nx) * p.ny * p.nz;
+    const int64_t n = static_cast<int64_t>(p.nx) * p.ny * p.nz;
+    if (p.qstride < n) p.qstride = n;  // a plane stride below the cell count means "packed"
     if (int rc = check_params(&p, d.obstacle)) return rc;
-    if (p.qstride >= (1 << 24))
+    if (n >= (1 << 24))
       return fail(TLBM_ERR_INVALID, "domain %d has %lld cells; batched domains hold < 2^24", i,
-                  static_cast<long long>(p.qstride));
+                  static_cast<long long>(n));
   }
   tlbm_batch* b = new tlbm_batch();
   b->handle = handle - d.diag;
   b->spec = d;
   (void)hipGetDevice(&b->device);
   b->params.assign(params, params + ndomains);
-  for (auto& p : b->params) p.qstride = static_cast<int64_t>(p.nx) * p.ny * p.nz;
+  for (auto& p : b->params)
+    p.qstride = std::max<int64_t>(p.qstride, static_cast<int64_t>(p.nx) * p.ny * p.nz);
   if (int rc = batch_build(b)) {
     batch_free_tables(b);
     delete b;

@@ -25,18 +25,23 @@ def _domains():
             Domain(8, 6, 4, 100.0, 3.0)]
 
 
+def _view(raw, batch, d):
+    """Domain d's (19, n) populations inside a raw batch buffer (plane stride of the batch)."""
+    e, _ = batch.offsets(d)
+    n, qs = batch.domains[d].n, batch.plane_strides[d]
+    return raw[e:e + 19 * qs].reshape(19, qs)[:, :n]
+
+
 def _random_state(batch, seed):
     import tlbm
     rng = np.random.default_rng(seed)
-    raw = np.empty(batch.elements, batch.f[0].dtype)
+    raw = np.zeros(batch.elements, batch.f[0].dtype)
     for d, dom in enumerate(batch.domains):
-        e, _ = batch.offsets(d)
         rho = 1 + 0.01 * rng.standard_normal(dom.n)
         u = 0.05 * rng.standard_normal((3, dom.n))
         f = np.stack([O.feq(q, rho, *u) for q in range(19)]) + 1e-3 * rng.standard_normal(
             (19, dom.n))
-        raw[e:e + 19 * dom.n] = tlbm.encode_populations(f.astype(np.float32),
-                                                        batch.storage).reshape(-1)
+        _view(raw, batch, d)[:] = tlbm.encode_populations(f.astype(np.float32), batch.storage)
     batch.f[0].copy_from(raw)
     return raw
 
@@ -46,8 +51,7 @@ def _single_step(batch, d, raw, report):
     import tlbm
     dom = batch.domains[d]
     lat = tlbm.D3Q19(dom.nx, dom.ny, dom.nz)
-    e, _ = batch.offsets(d)
-    a = tlbm.array(raw[e:e + 19 * dom.n], batch.storage)
+    a = tlbm.array(np.ascontiguousarray(_view(raw, batch, d)).reshape(-1), batch.storage)
     out = tlbm.empty((19 * dom.n,), batch.storage)
     fields = {k: tlbm.zeros((dom.n,)) for k in tlbm.FIELDS}
     sums = tlbm.HostBuffer(14)
@@ -60,19 +64,20 @@ def _single_step(batch, d, raw, report):
 
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
 @pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
-def test_batch_step_matches_single_lattice_and_oracle(storage, collision):
+@pytest.mark.parametrize("plane_pad", [256, 0])
+def test_batch_step_matches_single_lattice_and_oracle(storage, collision, plane_pad):
     import tlbm
     from tlbm.batch import BatchSolver
-    b = BatchSolver(_domains(), collision=collision, storage=storage, cs=0.1, magic=0.25)
+    b = BatchSolver(_domains(), collision=collision, storage=storage, cs=0.1, magic=0.25,
+                    plane_pad=plane_pad)
     raw = _random_state(b, seed=5)
     b.step(report=True)
     tlbm.sync()
     got = b.f[0].numpy()
     sums = b.sums.numpy.reshape(len(b.domains), 14)
     for d, dom in enumerate(b.domains):
-        e, _ = b.offsets(d)
         single, ssums = _single_step(b, d, raw, report=True)
-        a32 = tlbm.decode_populations(got[e:e + 19 * dom.n].reshape(19, -1), storage)
+        a32 = tlbm.decode_populations(_view(got, b, d), storage)
         s32 = tlbm.decode_populations(single.reshape(19, -1), storage)
         diff = float(np.abs(a32 - s32).max())
         print(f"domain {d}: max |batch - single| = {diff:.3e}")
@@ -91,8 +96,8 @@ def test_batch_step_matches_single_lattice_and_oracle(storage, collision):
             phys = {"bgk": O.BGK, "trt": lambda nu: O.TRT(nu, 0.25, 0.1),
                     "les": lambda nu: O.LES(nu, 0.1)}[collision](b.u0 * dom.diameter / dom.reynolds)
             ref = O.step((dom.nx, dom.ny, dom.nz), phys, geo, O.Flow(b.u0, 0.75 * dom.nx, 1.0),
-                         raw[e:e + 19 * dom.n].astype(np.float64).reshape(19, dom.n))
-            err = np.abs(got[e:e + 19 * dom.n].reshape(19, -1) - ref).max() / np.abs(ref).max()
+                         _view(raw, b, d).astype(np.float64))
+            err = np.abs(_view(got, b, d) - ref).max() / np.abs(ref).max()
             assert err < 1e-6, (d, err)
 
 
