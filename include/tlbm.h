@@ -198,8 +198,8 @@ int tlbm_vortex_fields(const float* ux, const float* uy, const float* uz, const 
 
 /* ---- adjoint (no counterpart in the reference) --------------------------------------------
  * One reverse step of the discrete adjoint of the BGK step with a TLBM_FIELD obstacle, periodic
- * in x, y and z, FP32 populations, qstride = nx*ny*nz (params as for that forward step; phi is
- * the solid fraction).  The objective is J = sum_t lam_t * (x-momentum exchanged with the solid
+ * in x, y and z, FP32 populations; f_t, f_t1, mu_in and mu_out have plane stride params->qstride
+ * (params as for that forward step; phi is the solid fraction).  The objective is J = sum_t lam_t * (x-momentum exchanged with the solid
  * at step t) (lbm.py:429-430 summed over cells).  Given mu_in = dJ/dg^{t+1} (g: the pulled and
  * bounced populations before collision) and the states f_t, f_t1 of steps t and t + 1, writes
  * mu_out = dJ/dg^t and adds dJ/dphi of the cells of the parameter block

@@ -880,16 +880,13 @@ int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* 
     return fail(TLBM_ERR_INVALID, "mu_in and mu_out must be distinct buffers");
   if (!block_origin || !block_size || !gphi)
     return fail(TLBM_ERR_INVALID, "parameter block and its gradient buffer are required");
-  const uint64_t n = static_cast<uint64_t>(params->nx) * params->ny * params->nz;
-  if (static_cast<uint64_t>(params->qstride) != n)
-    return fail(TLBM_ERR_INVALID, "the adjoint needs qstride == nx*ny*nz");
   tlbm::AdjArgs a{};
   a.f_t = f_t;
   a.f_t1 = f_t1;
   a.mu_in = mu_in;
   a.mu_out = mu_out;
   a.phi = params->phi;
-  a.ns = n;
+  a.ns = static_cast<uint64_t>(params->qstride);  // >= n (check_params)
   a.nx = params->nx;
   a.ny = params->ny;
   a.nz = params->nz;

@@ -140,14 +140,17 @@ class SolidFractionAdjoint:
                          PeriodicY())
         self.phi = zeros((self.n,))
         self.geo = SolidField(self.phi)
-        self.params = make_params(self.lat, self.phys, self.geo, self.flow)
+        # population planes padded past a 64-cell boundary (power-of-two plane sizes stream
+        # slower on MI355X); all state and adjoint buffers share this stride
+        self.qstride = self.n if self.n < 4096 else (self.n + 63) // 64 * 64 + 256
+        self.params = make_params(self.lat, self.phys, self.geo, self.flow, self.qstride)
         self.quiet = kernel_handle(_capi.BGK, _capi.FIELD, _capi.QUIET, _capi.PERIODIC_Y,
                                    _capi.F32)
         self.report = kernel_handle(_capi.BGK, _capi.FIELD, _capi.REPORT, _capi.PERIODIC_Y,
                                     _capi.F32)
         nseg = (self.steps + self.k - 1) // self.k
-        self.ckpt = [DeviceArray(19 * self.n) for _ in range(nseg)]
-        self.f = [DeviceArray(19 * self.n), DeviceArray(19 * self.n)]
+        self.ckpt = [DeviceArray(19 * self.qstride) for _ in range(nseg)]
+        self.f = [DeviceArray(19 * self.qstride), DeviceArray(19 * self.qstride)]
         self.sums = HostBuffer(self.window * _SUMS)
         self._seg = None
         self._mu = None
@@ -199,7 +202,9 @@ class SolidFractionAdjoint:
             f0 = np.ascontiguousarray(f0, np.float32).reshape(-1)
             if f0.size != 19 * self.n:
                 raise ValueError(f"f0 has {f0.size} values, the lattice needs {19 * self.n}")
-            self.f[0].copy_from(f0)
+            padded = np.zeros((19, self.qstride), np.float32)
+            padded[:, :self.n] = f0.reshape(19, self.n)
+            self.f[0].copy_from(padded)
         first = self.steps - self.window
         t0 = _event(self.stream)
         for t in range(self.steps):
@@ -220,8 +225,8 @@ class SolidFractionAdjoint:
         if not self._ran:
             raise RuntimeError("backward() needs a forward() first")
         if self._seg is None:
-            self._seg = [DeviceArray(19 * self.n) for _ in range(min(self.k, self.steps))]
-            self._mu = [DeviceArray(19 * self.n), DeviceArray(19 * self.n)]
+            self._seg = [DeviceArray(19 * self.qstride) for _ in range(min(self.k, self.steps))]
+            self._mu = [DeviceArray(19 * self.qstride), DeviceArray(19 * self.qstride)]
             self._gphi = _F64(int(np.prod(self.size)))
         self._mu[0].zero_(self.stream)
         self._gphi.zero_(self.stream)

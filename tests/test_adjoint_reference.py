@@ -179,7 +179,7 @@ def test_adjoint_entry_point_rejects_bad_arguments():
     bad = (ctypes.c_int32 * 3)(8, 0, 4)
     assert L.tlbm_adjoint_step(ctypes.byref(p), dummy, dummy, dummy, ctypes.c_void_p(32), 0.0,
                                0.0, o, bad, dummy, 0, None) != 0
-    p.qstride = 8 * 6 * 4 + 16
+    p.qstride = 8 * 6 * 4 - 16                                            # below the cells
     assert L.tlbm_adjoint_step(ctypes.byref(p), dummy, dummy, dummy, ctypes.c_void_p(32), 0.0,
                                0.0, o, s, dummy, 0, None) != 0
     assert "qstride" in tlbm._capi.lib().tlbm_last_error().decode()
