@@ -236,12 +236,12 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
   return a;
 }
 
-// Threads per block along x: the smallest of 64/128 that covers a row, else 256 for FP32 and
-// 128 for the 16-bit formats (the fastest shapes measured at 512 cells per row).
-int default_block_x(int nx, int storage) {
+// Threads per block along x: the smallest of 64/128 that covers a row, else 256 (the fastest
+// shape measured at 512 cells per row for every storage format, with padded population planes).
+int default_block_x(int nx, int) {
   if (nx <= 64) return 64;
   if (nx <= 128) return 128;
-  return storage == TLBM_F32 ? 256 : 128;
+  return 256;
 }
 
 template <int O, int S>
