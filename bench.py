@@ -299,8 +299,8 @@ def time_derived(reps=20):
     tlbm.sync(stream.value)
     ms, step_ms = ev.elapsed(e0, e1) / reps, ev.elapsed(e1, e2) / reps
     m, cells = d.n, d.cells
-    # gradient: 3 velocity reads + 9 writes per point; cells: 9 gradient reads + 2 writes per cell
-    algorithmic = 4 * (12 * m + 11 * cells)
+    # one pass: 3 velocity reads per point, 2 writes per cell
+    algorithmic = 4 * (3 * m + 2 * cells)
     del d, s
     L.tlbm_stream_destroy(stream)
     return {"grid": f"{NX}x{NY}x{NZ} points", "ms": ms, "step_ms": step_ms,

@@ -188,13 +188,12 @@ int tlbm_velocity_gradient(const float* ux, const float* uy, const float* uz, fl
                            int my, int mz, float inv_h, void* stream);
 
 /* |curl u| and the Q-criterion at every cell centre (the default derived fields,
- * derived.py:26-29).  basis = 0: the cell's gradient is the mean of its 8 corners' point
- * gradients (jac from tlbm_velocity_gradient); basis = 1: the trilinear cell's gradient at its
- * centre, the mean of the four edge differences per axis (velocity arrays).  Either output may
- * be NULL. */
-int tlbm_vortex_fields(const float* ux, const float* uy, const float* uz, const float* jac,
-                       float* vorticity, float* qcrit, int mx, int my, int mz, float inv_h,
-                       int basis, void* stream);
+ * derived.py:26-29), from the velocity in one pass.  basis = 0: the cell's gradient is the mean
+ * of its 8 corners' point gradients (those of tlbm_velocity_gradient, formed on the fly);
+ * basis = 1: the trilinear cell's gradient at its centre, the mean of the four edge differences
+ * per axis.  Either output may be NULL; outputs hold (mx - 1)(my - 1)(mz - 1) floats. */
+int tlbm_vortex_fields(const float* ux, const float* uy, const float* uz, float* vorticity,
+                       float* qcrit, int mx, int my, int mz, float inv_h, int basis, void* stream);
 
 /* ---- adjoint (no counterpart in the reference) --------------------------------------------
  * One reverse step of the discrete adjoint of the BGK step with a TLBM_FIELD obstacle, periodic

@@ -47,67 +47,150 @@ __global__ void __launch_bounds__(256)
 }
 
 // |curl u| and Q = -1/2 (G_00^2 + G_11^2 + G_22^2) - (G_01 G_10 + G_02 G_20 + G_12 G_21) at each
-// cell centre, with G the cell's velocity gradient: basis = 0, the mean of its eight corners'
-// point gradients (the solver's gradient); basis = 1, the trilinear cell's gradient at its
-// centre, the mean of the four edge differences along each axis.
-__global__ void __launch_bounds__(256)
+// cell centre, with G the cell's velocity gradient: BASIS = false, the mean of its eight
+// corners' point gradients (the solver's gradient, the points' as velocity_gradient_kernel forms
+// them); BASIS = true, the trilinear cell's gradient at its centre, the mean of the four edge
+// differences along each axis.  Both read the velocity only: no point gradient goes through HBM.
+//
+// Mapping.  A block is 64 x 4 cell columns (one wave per y row) and a chunk of kz cells along z;
+// each thread marches up its column plane by plane.  Per point plane z it forms, for each
+// component, three sums over its 2 x 2 corner points (x in {i, i+1}, y in {j, j+1}):
+//   px(z): the x-derivatives (lattice) or the x edge differences (basis),
+//   py(z): the same along y,
+//   s(z):  the four values, whose differences across planes give the z-derivatives;
+// a cell k then takes planes k and k + 1 (lattice z: s at planes k - 1 .. k + 2, one-sided at
+// the grid's faces).  The x and y neighbours come from the cache lines the wave's own loads and
+// the neighbouring rows' loads bring in; blocks go to the XCDs in contiguous runs of tiles so
+// rows that share lines share an L2.
+constexpr int kCellTileX = 64;
+constexpr int kCellTileY = 4;
+constexpr int kCellChunkZ = 32;
+
+struct CellColumn {
+  // offsets within a plane of the corner rows' points and of the y neighbour rows
+  uint32_t rj, rj1, ryl, ryh;
+  uint32_t i, xl, xh1;   // x of the corner column i, its left neighbour, i + 1's right neighbour
+  float sx0, sx1, sy0, sy1;
+};
+
+template <bool BASIS>
+__device__ __forceinline__ void plane_sums(const float* __restrict__ v, const CellColumn& t,
+                                           float& px, float& py, float& s) {
+  const float b0 = v[t.rj + t.i], c0 = v[t.rj + t.i + 1];
+  const float b1 = v[t.rj1 + t.i], c1 = v[t.rj1 + t.i + 1];
+  s = (b0 + c0) + (b1 + c1);
+  if (BASIS) {
+    px = (c0 - b0) + (c1 - b1);
+    py = (b1 - b0) + (c1 - c0);
+  } else {
+    const float a0 = v[t.rj + t.xl], d0 = v[t.rj + t.xh1];
+    const float a1 = v[t.rj1 + t.xl], d1 = v[t.rj1 + t.xh1];
+    const float e0 = v[t.ryl + t.i], e1 = v[t.ryl + t.i + 1];
+    const float h0 = v[t.ryh + t.i], h1 = v[t.ryh + t.i + 1];
+    // d/dx at corners (i, y) and (i + 1, y), y = j, j + 1; d/dy at corners (x, j), (x, j + 1)
+    px = ((c0 - a0) + (c1 - a1)) * t.sx0 + ((d0 - b0) + (d1 - b1)) * t.sx1;
+    py = ((b1 - e0) + (c1 - e1)) * t.sy0 + ((h0 - b0) + (h1 - c0)) * t.sy1;
+  }
+}
+
+__device__ __forceinline__ float corner_sum(const float* __restrict__ v, const CellColumn& t) {
+  return (v[t.rj + t.i] + v[t.rj + t.i + 1]) + (v[t.rj1 + t.i] + v[t.rj1 + t.i + 1]);
+}
+
+template <bool BASIS>
+__global__ void __launch_bounds__(kCellTileX * kCellTileY)
     vortex_fields_kernel(const float* __restrict__ ux, const float* __restrict__ uy,
-                         const float* __restrict__ uz, const float* __restrict__ jac,
-                         float* __restrict__ vorticity, float* __restrict__ qcrit, int mx, int my,
-                         int mz, float inv_h, int basis) {
+                         const float* __restrict__ uz, float* __restrict__ vorticity,
+                         float* __restrict__ qcrit, int mx, int my, int mz, float inv_h,
+                         uint32_t xtiles, uint32_t ytiles, uint32_t tiles, uint32_t per_xcd) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t tile = (b % 8u) * per_xcd + b / 8u;   // XCD b % 8 takes a contiguous run
+  if (tile >= tiles) return;
   const int cx = mx - 1, cy = my - 1, cz = mz - 1;
-  const uint32_t nc = static_cast<uint32_t>(cx) * cy * cz;
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nc) return;
-  const int i = static_cast<int>(e % static_cast<uint32_t>(cx));
-  const int j = static_cast<int>((e / static_cast<uint32_t>(cx)) % static_cast<uint32_t>(cy));
-  const int k = static_cast<int>(e / (static_cast<uint32_t>(cx) * cy));
-  const uint32_t m = static_cast<uint32_t>(mx) * my * mz;
-  auto pt = [&](int di, int dj, int dk) {
-    return static_cast<uint32_t>(i + di) +
-           mx * (static_cast<uint32_t>(j + dj) + my * static_cast<uint32_t>(k + dk));
-  };
-  float G[3][3];
-  if (basis) {
-    const float* const u[3] = {ux, uy, uz};
-    const float q = 0.25f * inv_h;
+  const uint32_t tx = tile % xtiles, ty = (tile / xtiles) % ytiles, tz = tile / (xtiles * ytiles);
+  const int i = static_cast<int>(tx) * kCellTileX + static_cast<int>(threadIdx.x % kCellTileX);
+  const int j = static_cast<int>(ty) * kCellTileY + static_cast<int>(threadIdx.x / kCellTileX);
+  if (i >= cx || j >= cy) return;
+  const int k0 = static_cast<int>(tz) * kCellChunkZ;
+  const int k1 = min(k0 + kCellChunkZ, cz);           // cells k0 .. k1 - 1, corner planes k0 .. k1
+  CellColumn t;
+  const uint32_t umx = static_cast<uint32_t>(mx);
+  const int yl = j > 0 ? j - 1 : 0, yh1 = j + 2 < my ? j + 2 : j + 1;
+  t.i = static_cast<uint32_t>(i);
+  t.xl = static_cast<uint32_t>(i > 0 ? i - 1 : 0);
+  t.xh1 = static_cast<uint32_t>(i + 2 < mx ? i + 2 : i + 1);
+  t.rj = static_cast<uint32_t>(j) * umx;
+  t.rj1 = t.rj + umx;
+  t.ryl = static_cast<uint32_t>(yl) * umx;
+  t.ryh = static_cast<uint32_t>(yh1) * umx;
+  t.sx0 = inv_h / static_cast<float>(i + 1 - static_cast<int>(t.xl));
+  t.sx1 = inv_h / static_cast<float>(static_cast<int>(t.xh1) - i);
+  t.sy0 = inv_h / static_cast<float>(j + 1 - yl);
+  t.sy1 = inv_h / static_cast<float>(yh1 - j);
+  const uint32_t plane = umx * static_cast<uint32_t>(my);
+  const float* const u[3] = {ux, uy, uz};
+  const float q = BASIS ? 0.25f * inv_h : 0.125f;
+
+  float px[3], py[3], s[3];        // plane kk
+  float ppx[3], ppy[3], pz[3];     // plane kk - 1 (and its z-derivative sum, lattice)
+  float sm[3];                     // s at plane max(kk - 1, 0) (lattice)
+#pragma unroll
+  for (int c = 0; c < 3; ++c) plane_sums<BASIS>(u[c] + k0 * plane, t, px[c], py[c], s[c]);
+  if (!BASIS) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sm[c] = k0 > 0 ? corner_sum(u[c] + (k0 - 1) * plane, t) : s[c];
+  }
+  uint32_t e = static_cast<uint32_t>(i) +
+               static_cast<uint32_t>(cx) * (static_cast<uint32_t>(j) + static_cast<uint32_t>(cy) *
+                                                                        static_cast<uint32_t>(k0));
+  const uint32_t cplane = static_cast<uint32_t>(cx) * static_cast<uint32_t>(cy);
+  for (int kk = k0; kk <= k1; ++kk) {
+    float nx[3], ny[3], ns[3], zs[3];
+    if (kk < k1) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        plane_sums<BASIS>(u[c] + (kk + 1) * plane, t, nx[c], ny[c], ns[c]);
+    } else if (!BASIS) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        nx[c] = ny[c] = 0.0f;
+        ns[c] = kk + 1 < mz ? corner_sum(u[c] + (kk + 1) * plane, t) : s[c];
+      }
+    }
+    if (!BASIS) {
+      const float sz = inv_h / static_cast<float>(min(kk + 1, mz - 1) - max(kk - 1, 0));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) zs[c] = (ns[c] - sm[c]) * sz;
+    }
+    if (kk > k0) {
+      float G[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        G[c][0] = q * (ppx[c] + px[c]);
+        G[c][1] = q * (ppy[c] + py[c]);
+        G[c][2] = BASIS ? q * (s[c] - pz[c]) : q * (pz[c] + zs[c]);
+      }
+      if (vorticity) {
+        const float wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
+        vorticity[e] = sqrtf(wx * wx + wy * wy + wz * wz);
+      }
+      if (qcrit) {
+        qcrit[e] = -0.5f * (G[0][0] * G[0][0] + G[1][1] * G[1][1] + G[2][2] * G[2][2]) -
+                   (G[0][1] * G[1][0] + G[0][2] * G[2][0] + G[1][2] * G[2][1]);
+      }
+      e += cplane;
+    }
+    if (kk == k1) break;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const float* v = u[c];
-      float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          gx += v[pt(1, a, b)] - v[pt(0, a, b)];
-          gy += v[pt(a, 1, b)] - v[pt(a, 0, b)];
-          gz += v[pt(a, b, 1)] - v[pt(a, b, 0)];
-        }
-      G[c][0] = gx * q;
-      G[c][1] = gy * q;
-      G[c][2] = gz * q;
+      ppx[c] = px[c];
+      ppy[c] = py[c];
+      pz[c] = BASIS ? s[c] : zs[c];
+      if (!BASIS) sm[c] = s[c];
+      px[c] = nx[c];
+      py[c] = ny[c];
+      s[c] = ns[c];
     }
-  } else {
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const float* plane = jac + static_cast<uint64_t>(r) * m;
-      float s = 0.0f;
-#pragma unroll
-      for (int dk = 0; dk < 2; ++dk)
-#pragma unroll
-        for (int dj = 0; dj < 2; ++dj)
-#pragma unroll
-          for (int di = 0; di < 2; ++di) s += plane[pt(di, dj, dk)];
-      G[r / 3][r % 3] = 0.125f * s;
-    }
-  }
-  if (vorticity) {
-    const float wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
-    vorticity[e] = sqrtf(wx * wx + wy * wy + wz * wz);
-  }
-  if (qcrit) {
-    qcrit[e] = -0.5f * (G[0][0] * G[0][0] + G[1][1] * G[1][1] + G[2][2] * G[2][2]) -
-               (G[0][1] * G[1][0] + G[0][2] * G[2][0] + G[1][2] * G[2][1]);
   }
 }
 
@@ -137,20 +220,29 @@ int tlbm_velocity_gradient(const float* ux, const float* uy, const float* uz, fl
   return TLBM_OK;
 }
 
-int tlbm_vortex_fields(const float* ux, const float* uy, const float* uz, const float* jac,
-                       float* vorticity, float* qcrit, int mx, int my, int mz, float inv_h,
-                       int basis, void* stream) {
+int tlbm_vortex_fields(const float* ux, const float* uy, const float* uz, float* vorticity,
+                       float* qcrit, int mx, int my, int mz, float inv_h, int basis, void* stream) {
   if (mx < 2 || my < 2 || mz < 2) return fail_message("cell fields need >= 2 points per axis");
-  if (basis ? (!ux || !uy || !uz) : !jac)
-    return fail_message(basis ? "basis gradient needs the velocity arrays"
-                                    : "lattice gradient needs the point gradient (jac)");
+  if (!ux || !uy || !uz) return fail_message("cell fields: NULL velocity array");
   if (!vorticity && !qcrit) return TLBM_OK;
   const uint64_t m = static_cast<uint64_t>(mx) * my * mz;
-  if (9 * m > 0xFFFFFFFFull) return fail_message("grid too large for the cell fields");
-  const uint64_t nc = static_cast<uint64_t>(mx - 1) * (my - 1) * (mz - 1);
-  hipLaunchKernelGGL(tlbm::vortex_fields_kernel, dim3(static_cast<uint32_t>((nc + 255) / 256)),
-                     dim3(256), 0, static_cast<hipStream_t>(stream), ux, uy, uz, jac, vorticity,
-                     qcrit, mx, my, mz, inv_h, basis);
+  if (m > 0xFFFFFFFFull) return fail_message("grid too large for the cell fields");
+  const uint64_t xtiles = (mx - 1 + tlbm::kCellTileX - 1) / tlbm::kCellTileX;
+  const uint64_t ytiles = (my - 1 + tlbm::kCellTileY - 1) / tlbm::kCellTileY;
+  const uint64_t ztiles = (mz - 1 + tlbm::kCellChunkZ - 1) / tlbm::kCellChunkZ;
+  const uint64_t tiles = xtiles * ytiles * ztiles;
+  const uint64_t per_xcd = (tiles + 7) / 8;
+  if (8 * per_xcd > 0xFFFFFFFFull) return fail_message("grid too large for the cell fields");
+  const dim3 grid(static_cast<uint32_t>(8 * per_xcd)), block(tlbm::kCellTileX * tlbm::kCellTileY);
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const uint32_t a = static_cast<uint32_t>(xtiles), c = static_cast<uint32_t>(ytiles),
+                 t = static_cast<uint32_t>(tiles), p = static_cast<uint32_t>(per_xcd);
+  if (basis)
+    hipLaunchKernelGGL(tlbm::vortex_fields_kernel<true>, grid, block, 0, st, ux, uy, uz,
+                       vorticity, qcrit, mx, my, mz, inv_h, a, c, t, p);
+  else
+    hipLaunchKernelGGL(tlbm::vortex_fields_kernel<false>, grid, block, 0, st, ux, uy, uz,
+                       vorticity, qcrit, mx, my, mz, inv_h, a, c, t, p);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(e, "cell fields launch");
   return TLBM_OK;

@@ -98,8 +98,8 @@ SIGNATURES = {
                                   _c.POINTER(Heat), _c.POINTER(Fields), _vp, _vp]),
     "tlbm_velocity_gradient": (_c.c_int, [_vp, _vp, _vp, _vp, _c.c_int, _c.c_int, _c.c_int,
                                           _c.c_float, _vp]),
-    "tlbm_vortex_fields": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c.c_int, _c.c_int,
-                                      _c.c_int, _c.c_float, _c.c_int, _vp]),
+    "tlbm_vortex_fields": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_int, _c.c_int, _c.c_int,
+                                      _c.c_float, _c.c_int, _vp]),
     "tlbm_adjoint_step": (_c.c_int, [_c.POINTER(Params), _vp, _vp, _vp, _vp, _c.c_float,
                                      _c.c_float, _c.POINTER(_c.c_int32),
                                      _c.POINTER(_c.c_int32), _vp, _c.c_int, _vp]),

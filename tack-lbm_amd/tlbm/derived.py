@@ -3,7 +3,7 @@
 The reference evaluates VTK-calculator expressions (tack-expr) over the output grid; its two
 default fields are the vorticity magnitude ``mag(curl(velocity))`` and the Q-criterion
 ``qcrit(velocity)``, both at cell centres (derived.py:26-29).  This module computes those two
-with hand-written kernels (``tlbm_velocity_gradient``, ``tlbm_vortex_fields``), with the
+with one hand-written kernel (``tlbm_vortex_fields``, reading the velocity once), with the
 solver's gradient (central differences at the points, one-sided at the faces, a cell taking the
 mean of its eight corners; ``gradient="lattice"``, the default) or the trilinear basis's
 (``gradient="basis"``).  Other expressions need the expression compiler, which is not part of
@@ -79,7 +79,6 @@ class DerivedFields:
             self._vel = [empty((self.n,)) for _ in range(3)]
         else:
             self._vel = None
-        self.jacobian = empty((9 * self.n,)) if gradient == "lattice" else None
         self._out = {k: empty((self.cells,)) for k in set(self.kinds.values())}
         # the steering status the Catalyst channel reports (derived.py:113-115); no steered
         # expressions without the expression compiler
@@ -100,18 +99,13 @@ class DerivedFields:
         return self._vel
 
     def compute(self):
-        """Launch the kernels on the solver's stream (device results in ``self._out``)."""
+        """Launch the kernel on the solver's stream (device results in ``self._out``)."""
         L = lib()
         u = self._velocity()
         st = self.solver.stream
-        if self.jacobian is not None:
-            check(L.tlbm_velocity_gradient(u[0].ptr, u[1].ptr, u[2].ptr, self.jacobian.ptr,
-                                           self.mx, self.my, self.mz, self.inv_h, st),
-                  "velocity gradient")
         vort = self._out.get("vorticity")
         qc = self._out.get("qcrit")
         check(L.tlbm_vortex_fields(u[0].ptr, u[1].ptr, u[2].ptr,
-                                   self.jacobian.ptr if self.jacobian is not None else None,
                                    vort.ptr if vort is not None else None,
                                    qc.ptr if qc is not None else None,
                                    self.mx, self.my, self.mz, self.inv_h,

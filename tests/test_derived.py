@@ -78,9 +78,10 @@ def test_entry_points_reject_bad_arguments():
     p = ctypes.c_void_p(64)
     assert L.tlbm_velocity_gradient(None, p, p, p, 4, 4, 4, 1.0, None) != 0
     assert L.tlbm_velocity_gradient(p, p, p, p, 0, 4, 4, 1.0, None) != 0
-    assert L.tlbm_vortex_fields(p, p, p, None, p, p, 4, 4, 4, 1.0, 0, None) != 0   # no jac
-    assert L.tlbm_vortex_fields(None, p, p, p, p, p, 4, 4, 4, 1.0, 1, None) != 0   # no u
-    assert L.tlbm_vortex_fields(p, p, p, p, p, p, 1, 4, 4, 1.0, 0, None) != 0      # no cells
+    for basis in (0, 1):
+        assert L.tlbm_vortex_fields(None, p, p, p, p, 4, 4, 4, 1.0, basis, None) != 0  # no u
+        assert L.tlbm_vortex_fields(p, p, p, p, p, 1, 4, 4, 1.0, basis, None) != 0     # no cells
+        assert L.tlbm_vortex_fields(p, p, p, None, None, 4, 4, 4, 1.0, basis, None) == 0
 
 
 # ---- GPU ---------------------------------------------------------------------------------------
@@ -95,17 +96,26 @@ def _gpu_fields(u32, h, basis):
     vort, qc = tlbm.empty((cells,)), tlbm.empty((cells,))
     tlbm._capi.check(L.tlbm_velocity_gradient(dev[0].ptr, dev[1].ptr, dev[2].ptr, jac.ptr, mx,
                                               my, mz, 1.0 / h, None), "gradient")
-    tlbm._capi.check(L.tlbm_vortex_fields(dev[0].ptr, dev[1].ptr, dev[2].ptr, jac.ptr, vort.ptr,
-                                          qc.ptr, mx, my, mz, 1.0 / h, basis, None), "cells")
+    tlbm._capi.check(L.tlbm_vortex_fields(dev[0].ptr, dev[1].ptr, dev[2].ptr, vort.ptr, qc.ptr,
+                                          mx, my, mz, 1.0 / h, basis, None), "cells")
     tlbm.sync()
+    # one output only: the other is not written
+    only = tlbm.empty((cells,))
+    tlbm._capi.check(L.tlbm_vortex_fields(dev[0].ptr, dev[1].ptr, dev[2].ptr, None, only.ptr,
+                                          mx, my, mz, 1.0 / h, basis, None), "q only")
+    tlbm.sync()
+    assert np.array_equal(only.numpy(), qc.numpy())
     return (jac.numpy().reshape(3, 3, mz, my, mx), vort.numpy().reshape(mz - 1, my - 1, mx - 1),
             qc.numpy().reshape(mz - 1, my - 1, mx - 1))
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dims", [(2, 2, 2), (7, 5, 3), (64, 33, 17), (130, 9, 4)])
+@pytest.mark.parametrize("dims", [(2, 2, 2), (7, 5, 3), (64, 33, 17), (130, 9, 4), (3, 2, 70),
+                                  (66, 7, 35), (2, 6, 33)])
 @pytest.mark.parametrize("basis", [0, 1])
 def test_kernels_match_oracle(dims, basis):
+    """Point gradients and cell fields against the oracle; the cell kernel's z chunks are 32
+    cells, so the last three grids cross chunk boundaries (x and y tiles: 64 x 4 cells)."""
     rng = np.random.default_rng(sum(dims) + basis)
     mx, my, mz = dims
     u = (0.05 * rng.standard_normal((3, mz, my, mx))).astype(np.float32)

@@ -148,6 +148,41 @@ def test_ragged_and_degenerate_grids(dims):
     assert _rel(got, ref) < 1e-6
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_planes_beyond_32_bit_offsets(storage):
+    """A lattice of 2^27 cells (64 x 64 x 32768): its population planes end beyond 2^31 elements
+    and beyond 2^32 bytes.  State, old fields and geometry (a rotating z-axis cylinder) repeat
+    every 4 cells along z, so one report step must give, bit for bit, the step of the 64 x 64 x 4
+    lattice repeated 8192 times, and FP64 sums 8192 times that lattice's."""
+    import tlbm
+    reps = 8192
+    small = Case((64, 64, 4), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
+    big = Case((64, 64, 4 * reps), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
+    assert 19 * big.n > 2 ** 31
+    raw = tlbm.encode_populations(random_populations(small.n, seed=31).astype(np.float32),
+                                  storage).reshape(19, 1, small.n)
+    old = random_fields(small.n, 32)
+    results = []
+    for case, r in ((small, 1), (big, reps)):
+        lat, phys, geo, flow = case.device()
+        f_in = tlbm.array(np.broadcast_to(raw, (19, r, small.n)), storage)
+        f_out = tlbm.empty((19 * case.n,), storage)
+        fields = {k: tlbm.array(np.broadcast_to(old[k], (r, small.n))) for k in tlbm.FIELDS}
+        sums = tlbm.HostBuffer(14)
+        tlbm.lbm_step(lat, phys, geo, flow, tlbm.Report(fields), f_in, f_out, sums=sums)
+        tlbm.sync()
+        del f_in
+        results.append((f_out, fields, sums.numpy.copy()))
+    (s_out, s_fields, s_sums), (b_out, b_fields, b_sums) = results
+    assert np.array_equal(b_out.numpy().reshape(19, reps, small.n),
+                          np.broadcast_to(s_out.numpy().reshape(19, 1, small.n),
+                                          (19, reps, small.n)))
+    for k in tlbm.FIELDS:
+        assert np.array_equal(b_fields[k].numpy().reshape(reps, small.n),
+                              np.broadcast_to(s_fields[k].numpy(), (reps, small.n))), k
+    assert np.all(np.abs(b_sums - reps * s_sums) <= 1e-9 * np.abs(reps * s_sums))
+
+
 @pytest.mark.parametrize("storage,bits", [("f16", 10), ("bf16", 7)])
 @pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
 @pytest.mark.parametrize("side", ["periodic", "slip", "lid"])
