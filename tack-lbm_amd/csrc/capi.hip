@@ -907,15 +907,20 @@ int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* 
                   block_origin[i], block_origin[i] + block_size[i], dims[i], i);
   }
   a.gphi = gphi;
-  const uint64_t xblocks = (params->nx + 255) / 256;
+  // rows of up to 128 cells take blocks of 64/128 threads, as the forward step does: a 256-thread
+  // block on a 128-cell row would leave half its waves idle
+  const int bx = default_block_x(params->nx, TLBM_F32);
+  const uint64_t xblocks = (params->nx + bx - 1) / bx;
   a.xblocks = static_cast<uint32_t>(xblocks);
   a.rows = static_cast<uint32_t>(static_cast<uint64_t>(params->ny) * params->nz);
-  const dim3 grid(static_cast<uint32_t>(xblocks * a.rows));
+  if (xblocks * a.rows * bx > 0xFFFFFFFFull)
+    return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
+  const dim3 grid(static_cast<uint32_t>(xblocks * a.rows)), block(bx);
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (final_links)
-    hipLaunchKernelGGL(tlbm::lbm_adjoint_kernel<true>, grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL(tlbm::lbm_adjoint_kernel<true>, grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL(tlbm::lbm_adjoint_kernel<false>, grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL(tlbm::lbm_adjoint_kernel<false>, grid, block, 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "adjoint launch");
   return TLBM_OK;
