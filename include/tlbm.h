@@ -122,8 +122,8 @@ int tlbm_event_elapsed_ms(void* start, void* stop, float* ms);
 /* Resolve a specialisation (Tack's type-keyed kernel cache, README.md:55-56).  The handle is
  * stable for the life of the process. */
 int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle);
-/* Launch shape of a specialisation: threads per block along x (64, 128 or 256; 0 restores the
- * default: the smallest of 64/128 covering a row, else 256 for FP32 and 128 for FP16/BF16). */
+/* Launch shape of a specialisation: threads per block along x (64, 128, 192 or 256; 0 restores
+ * the default: of those four, the one launching the fewest lanes per row, the larger on a tie). */
 int tlbm_set_launch(int handle, int block_x);
 
 /* lbm_init (lbm.py:383-394): f = feq(1, u0 fluid, seed_amp u0 fluid sin(2 pi x/nx)

@@ -102,7 +102,7 @@ def tune_batch(batch, reps=5, use_cache=True, candidates=None):
     return best
 
 
-def tune_solver(solver, reps=5, use_cache=True, blocks=(64, 128, 256)):
+def tune_solver(solver, reps=5, use_cache=True, blocks=(64, 128, 192, 256)):
     """Pick threads per block for a Solver's quiet kernel (the report kernel follows it)."""
     if getattr(solver, "heat_enabled", False):
         raise NotImplementedError("tune_solver times the flow step; this solver has heat")

@@ -139,7 +139,9 @@ def test_step_argument_errors_are_reported_before_any_launch():
     assert "report" in _capi.last_error()
     assert lib.tlbm_set_launch(h, 96) != 0
     assert lib.tlbm_set_launch(12345, 64) != 0
-    assert lib.tlbm_set_launch(h, 64) == 0 and lib.tlbm_set_launch(h, 0) == 0
+    assert lib.tlbm_set_launch(h, 320) != 0 and lib.tlbm_set_launch(h, -64) != 0
+    for bx in (64, 192, 0):
+        assert lib.tlbm_set_launch(h, bx) == 0
 
 
 def test_shifted_population_codecs():

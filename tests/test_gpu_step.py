@@ -93,9 +93,10 @@ def test_quiet_step_f32(collision, obstacle, side, block_x):
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
-@pytest.mark.parametrize("dims", [(256, 3, 2), (512, 3, 2), (1024, 2, 2), (96, 4, 3), (16, 5, 4)])
+@pytest.mark.parametrize("dims", [(256, 3, 2), (512, 3, 2), (1024, 2, 2), (96, 4, 3), (16, 5, 4),
+                                  (320, 3, 2), (200, 2, 3)])
 @pytest.mark.parametrize("side", ["periodic", "lid"])
-@pytest.mark.parametrize("block_x", [0, 64, 256])
+@pytest.mark.parametrize("block_x", [0, 64, 192, 256])
 def test_rows_across_blocks(storage, dims, side, block_x):
     """Rows of one to sixteen blocks (x streaming across block edges and the periodic wrap) and
     rows shorter than a block; the obstacle straddles x = 0."""
