@@ -230,7 +230,7 @@ int tlbm_batch_destroy(tlbm_batch* batch);
  * population elements, `cells` lattice cells over all domains.  Any output may be NULL. */
 int tlbm_batch_layout(const tlbm_batch* batch, int domain, uint64_t* elem_offset,
                       uint64_t* cell_offset, uint64_t* elements, uint64_t* cells);
-/* Threads per block (64, 128, 256; 0 = 256) and persistent grid size in blocks (0 = one block
+/* Threads per block (64, 128, 192, 256; 0 = 256) and persistent grid size in blocks (0 = one block
  * per chunk).  Changing block_x synchronises the device and rebuilds the chunk tables. */
 int tlbm_batch_set_launch(tlbm_batch* batch, int block_x, int grid_blocks);
 /* lbm_init of every domain (see tlbm_init). */

@@ -760,8 +760,8 @@ int tlbm_batch_layout(const tlbm_batch* b, int domain, uint64_t* elem_offset,
 int tlbm_batch_set_launch(tlbm_batch* b, int block_x, int grid_blocks) {
   if (!b) return fail(TLBM_ERR_INVALID, "batch is NULL");
   if (block_x == 0) block_x = 256;
-  if (block_x != 64 && block_x != 128 && block_x != 256)
-    return fail(TLBM_ERR_INVALID, "block_x must be 0, 64, 128 or 256 (got %d)", block_x);
+  if (block_x < 64 || block_x > 256 || block_x % 64 != 0)
+    return fail(TLBM_ERR_INVALID, "block_x must be 0, 64, 128, 192 or 256 (got %d)", block_x);
   if (grid_blocks < 0) return fail(TLBM_ERR_INVALID, "grid_blocks must be >= 0");
   b->grid = grid_blocks;
   if (block_x != b->block) {

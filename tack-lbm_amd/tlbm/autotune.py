@@ -80,7 +80,7 @@ def tune_batch(batch, reps=5, use_cache=True, candidates=None):
         return block, grid, ms
     if candidates is None:
         candidates = []
-        for block in (64, 128, 256):
+        for block in (64, 128, 192, 256):
             waves = block // 64
             for per_simd in (0, 2, 4, 8):  # 0: one block per chunk (not persistent)
                 grid = 0 if per_simd == 0 else cus.value * 4 * per_simd // waves

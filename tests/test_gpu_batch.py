@@ -102,20 +102,28 @@ def test_batch_step_matches_single_lattice_and_oracle(storage, collision, plane_
 
 
 @pytest.mark.parametrize("block,grid", [(64, 0), (128, 0), (256, 0), (64, 7), (256, 3),
-                                        (128, 1000)])
+                                        (128, 1000), (192, 0), (192, 5)])
 def test_batch_launch_shapes_agree(block, grid):
+    """Quiet and report steps give the same state and fields at every launch shape, and the
+    same per-domain FP64 sums up to summation order."""
     import tlbm
     from tlbm.batch import BatchSolver
-    b = BatchSolver(_domains(), collision="trt", storage="f32")
-    _random_state(b, seed=6)
-    b.step()
-    ref = b.f[0].numpy()
-    c = BatchSolver(_domains(), collision="trt", storage="f32")
-    c.set_launch(block, grid)
-    _random_state(c, seed=6)
-    c.step()
-    tlbm.sync()
-    assert np.array_equal(c.f[0].numpy(), ref)
+    out = []
+    for shape in ((0, 0), (block, grid)):
+        b = BatchSolver(_domains(), collision="trt", storage="f32", fields=True)
+        b.set_launch(*shape)
+        _random_state(b, seed=6)
+        b.step()
+        b.step(report=True)
+        tlbm.sync()
+        out.append((b.f[0].numpy(), {k: v.numpy() for k, v in b.fields.items()},
+                    b.sums.numpy.copy()))
+        b.close()
+    (f0, fl0, s0), (f1, fl1, s1) = out
+    assert np.array_equal(f1, f0)
+    for k in fl0:
+        assert np.array_equal(fl1[k], fl0[k]), k
+    assert np.allclose(s1, s0, rtol=1e-12, atol=1e-12 * np.abs(s0).max())
 
 
 def test_batch_graph_replays_match_eager_steps():
@@ -151,7 +159,7 @@ def test_batch_autotune_keeps_state(tmp_path, monkeypatch):
     _random_state(b, seed=7)
     before = b.f[0].numpy()
     block, grid, ms = autotune.tune_batch(b, reps=2)
-    assert block in (64, 128, 256) and ms > 0
+    assert block in (64, 128, 192, 256) and ms > 0
     assert np.array_equal(b.f[0].numpy(), before)
     again = autotune.tune_batch(b, reps=2)
     assert again[:2] == (block, grid)  # from the cache
