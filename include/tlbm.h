@@ -139,7 +139,8 @@ int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, fl
  * NULL, TLBM_NSUMS FP64 sums are written to sums[0..13] (device, managed or pinned host
  * memory) by the same stream.  A report step may pass fields = NULL when sums is not NULL:
  * it then writes only the sums, with the five residual sums zero.  `fields` and `sums` are
- * ignored by TLBM_QUIET handles.
+ * ignored by TLBM_QUIET handles.  Report steps on different streams may run concurrently:
+ * the library keeps one reduction workspace per (device, stream).
  * Replaces the host FP64 numpy sums of Solver.monitors (lbm.py:649-661). */
 int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* params,
               const tlbm_fields* fields, double* sums, void* stream);

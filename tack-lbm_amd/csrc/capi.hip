@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -103,20 +104,22 @@ bool decode(int h, Decoded& d) {
   return !(d.heat && d.storage != TLBM_F32);
 }
 
-// ---- per-device report workspace (grow-only) ----------------------------------------------
+// ---- report workspace, one per (device, stream), grow-only ----------------------------------
+// A report step's block partials live here until its reduction (same stream) has read them, so
+// report steps queued on different streams each need their own buffer; steps on one stream are
+// ordered by the stream.
 struct Workspace {
   double* ptr = nullptr;
   size_t bytes = 0;
 };
 std::mutex g_ws_mutex;
-std::vector<Workspace> g_ws;
+std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
 
-int workspace(size_t bytes, double** out) {
+int workspace(size_t bytes, hipStream_t stream, double** out) {
   int dev = 0;
   TLBM_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(g_ws_mutex);
-  if (static_cast<int>(g_ws.size()) <= dev) g_ws.resize(dev + 1);
-  Workspace& w = g_ws[dev];
+  Workspace& w = g_ws[{dev, stream}];
   if (w.bytes < bytes) {
     if (w.ptr) {
       TLBM_HIP(hipDeviceSynchronize());
@@ -626,7 +629,7 @@ int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const
     }
     if (sums) {
       double* ws = nullptr;
-      if (int rc = workspace(sizeof(double) * TLBM_NSUMS * nblocks, &ws)) return rc;
+      if (int rc = workspace(sizeof(double) * TLBM_NSUMS * nblocks, st, &ws)) return rc;
       a.partials = ws;
       a.nblocks = nblocks;
     }

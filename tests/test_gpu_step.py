@@ -184,6 +184,45 @@ def test_planes_beyond_32_bit_offsets(storage):
     assert np.all(np.abs(b_sums - reps * s_sums) <= 1e-9 * np.abs(reps * s_sums))
 
 
+def test_report_steps_on_two_streams_keep_their_own_sums():
+    """Report steps queued at once on two streams (a large lattice on one, a small one on the
+    other) give bit for bit the FP64 sums each gives when run alone."""
+    import ctypes
+    import tlbm
+    L = tlbm._capi.lib()
+    streams = []
+    for _ in range(2):
+        st = ctypes.c_void_p()
+        tlbm._capi.check(L.tlbm_stream_create(ctypes.byref(st)), "stream")
+        streams.append(st.value)
+
+    def solvers():
+        return (tlbm.Solver(256, 128, 96, collision="les", obstacle="sphere", reynolds=500.0,
+                            noise_amp=1e-3, stream=streams[0]),
+                tlbm.Solver(96, 48, 32, collision="trt", obstacle="cylinder", reynolds=200.0,
+                            noise_amp=1e-3, stream=streams[1]))
+    together, alone = [], []
+    a, b = solvers()
+    for _ in range(5):
+        a.step(report=True)
+        b.step(report=True)
+        for st in streams:
+            tlbm.sync(st)
+        together.append((a.sums.numpy.copy(), b.sums.numpy.copy()))
+    a2, b2 = solvers()
+    for _ in range(5):
+        a2.step(report=True)
+        tlbm.sync(streams[0])
+        b2.step(report=True)
+        tlbm.sync(streams[1])
+        alone.append((a2.sums.numpy.copy(), b2.sums.numpy.copy()))
+    for (ta, tb), (sa, sb) in zip(together, alone):
+        assert np.array_equal(ta, sa) and np.array_equal(tb, sb)
+    del a, b, a2, b2
+    for st in streams:
+        L.tlbm_stream_destroy(ctypes.c_void_p(st))
+
+
 @pytest.mark.parametrize("storage,bits", [("f16", 10), ("bf16", 7)])
 @pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
 @pytest.mark.parametrize("side", ["periodic", "slip", "lid"])
