@@ -43,6 +43,7 @@ def main():
                      lift=np.array(lift))
     np.savez(out, step=np.array(steps), drag=np.array(drag), lift=np.array(lift),
              sample_every=np.array(C2_SAMPLE_EVERY))
+    os.remove(out + ".partial.npz")
     print("wrote", out, flush=True)
 
 
