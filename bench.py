@@ -19,9 +19,12 @@ adjoint: the gradient workload (BGK 128x64x64, 2000 steps checkpointed every 100
 backward pass times and the adjoint kernel against the forward step kernel, under "adjoint".
 derived: vorticity magnitude and Q-criterion of the 512x256x256 output grid, under "derived".
 heat: the temperature variant's step against the flow step (heat-sink lattice), under "heat".
+graph: a small lattice stepped one step() at a time and by Solver.run hipGraph replays, under
+"graph".
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
-                       [--no-adjoint] [--no-derived] [--no-heat] [--storage f32,f16,bf16] [--block N]
+                       [--no-adjoint] [--no-derived] [--no-heat] [--no-graph]
+                       [--storage f32,f16,bf16] [--block N]
                        [--calibrate]
 """
 
@@ -307,6 +310,40 @@ def time_derived(reps=20):
             "achieved_gbs": algorithmic / (ms * 1e-3) / 1e9, "algorithmic_bytes": algorithmic}
 
 
+def time_graph(steps=1000, every=50):
+    """Host-launch cost on a small lattice: BGK 96x48x48 with a cylinder (r 6, Re 100), 1000
+    steps with a report step every 50, taken one step() at a time and as Solver.run(50, 50)
+    hipGraph replays; HIP events around each loop on the solver's stream."""
+    import tlbm
+    from tlbm._capi import check, lib
+    L = lib()
+    stream = ctypes.c_void_p()
+    check(L.tlbm_stream_create(ctypes.byref(stream)), "stream create")
+    s = tlbm.Solver(96, 48, 48, collision="bgk", obstacle="cylinder", reynolds=100.0,
+                    diameter=12.0, center=(24.0, 24.0, 24.0), noise_amp=1e-3,
+                    stream=stream.value)
+    s.run(every, every)                                  # capture + warm-up
+    ev = Events(L)
+    e0, e1, e2 = ev.new(), ev.new(), ev.new()
+    tlbm.sync(stream.value)
+    check(L.tlbm_event_record(e0, stream), "record")
+    for i in range(steps):
+        s.step(report=(i + 1) % every == 0)
+    check(L.tlbm_event_record(e1, stream), "record")
+    for _ in range(steps // every):
+        s.run(every, every)
+    check(L.tlbm_event_record(e2, stream), "record")
+    tlbm.sync(stream.value)
+    eager, graph = ev.elapsed(e0, e1), ev.elapsed(e1, e2)
+    cells = 96 * 48 * 48
+    del s
+    L.tlbm_stream_destroy(stream)
+    return {"lattice": "BGK 96x48x48, cylinder r6, Re 100, report every 50", "steps": steps,
+            "eager_ms_per_step": eager / steps, "graph_ms_per_step": graph / steps,
+            "eager_mlups": cells * steps / (eager * 1e-3) / 1e6,
+            "graph_mlups": cells * steps / (graph * 1e-3) / 1e6}
+
+
 def time_heat(reps=20):
     """The temperature variant (heat=True, lbm.py:457-492) on the headline lattice with a heat
     sink in place of the sphere (LES, 512x256x256, the Solver's heat-sink placement), against the
@@ -378,6 +415,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--no-derived", action="store_true", help="skip the derived-field timing")
     ap.add_argument("--no-heat", action="store_true", help="skip the temperature-variant timing")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the small-lattice step-by-step vs hipGraph timing")
     ap.add_argument("--calibrate", action="store_true",
                     help="also time a plain 1 GiB copy (counter calibration, streaming ceiling)")
     args = ap.parse_args()
@@ -417,6 +456,12 @@ def main():
         print(f"[bench] heat: step {heat['heat_step_ms']:.3f} ms vs flow {heat['flow_step_ms']:.3f} "
               f"ms ({heat['heat_over_flow']:.2f}x, {heat['achieved_gbs']:.0f} GB/s)",
               file=sys.stderr, flush=True)
+    graph = None
+    if not args.no_graph:
+        graph = time_graph()
+        print(f"[bench] small lattice (96x48x48): {graph['eager_mlups']:.0f} MLUPS step by step, "
+              f"{graph['graph_mlups']:.0f} MLUPS from hipGraph replays", file=sys.stderr,
+              flush=True)
     copy = calibrate_copy() if args.calibrate else None
     if copy:
         print(f"[bench] copy kernel: {copy['gbs']:.0f} GB/s", file=sys.stderr, flush=True)
@@ -460,6 +505,9 @@ def main():
                            for k, v in adjoint.items()}
     if heat:
         line["heat"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in heat.items()}
+    if graph:
+        line["graph"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                         for k, v in graph.items()}
     if derived:
         line["derived"] = {k: (round(v, 4) if isinstance(v, float) else v)
                            for k, v in derived.items()}

@@ -106,7 +106,8 @@ int tlbm_free(void* ptr);                      /* device or managed */
 int tlbm_free_host(void* ptr);
 int tlbm_memcpy(void* dst, const void* src, size_t bytes);                   /* synchronous  */
 int tlbm_memcpy_async(void* dst, const void* src, size_t bytes, void* stream);
-int tlbm_memset(void* ptr, int value, size_t bytes, void* stream);          /* asynchronous */
+int tlbm_memset(void* ptr, int value, size_t bytes, void* stream);  /* asynchronous; refused
+                                                                       under graph capture */
 
 /* ---- streams, events (replaces tack.sync, lbm.py:651) ------------------------------------ */
 int tlbm_stream_create(void** stream);
@@ -250,6 +251,15 @@ int tlbm_batch_graph(tlbm_batch* batch, void* f_a, void* f_b, int steps, int rep
                      const tlbm_fields* fields, double* sums, void* stream, tlbm_graph** out);
 int tlbm_graph_launch(tlbm_graph* graph, void* stream);
 int tlbm_graph_destroy(tlbm_graph* graph);
+/* Capture the library calls this thread issues on `stream` (a created stream, not the null
+ * stream) between tlbm_capture_begin and tlbm_capture_end into a graph (launch it with
+ * tlbm_graph_launch, on any stream of the device).  Steps, inits, copies and coarsening can
+ * be captured; nothing that allocates: the first report step with sums on a stream (or on a
+ * larger lattice than before) sizes its reduction workspace and must run eagerly first, and
+ * fails with TLBM_ERR_INVALID under capture; tlbm_memset is refused.  When a call fails under
+ * capture, end the capture and destroy what tlbm_capture_end returns. */
+int tlbm_capture_begin(void* stream);
+int tlbm_capture_end(void* stream, tlbm_graph** out);
 /* Compute units and architecture name of the current device (sizing persistent grids, keying
  * tuned launch shapes). */
 int tlbm_device_cus(int* cus);

@@ -121,6 +121,12 @@ int workspace(size_t bytes, hipStream_t stream, double** out) {
   std::lock_guard<std::mutex> lock(g_ws_mutex);
   Workspace& w = g_ws[{dev, stream}];
   if (w.bytes < bytes) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    TLBM_HIP(hipStreamIsCapturing(stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      return fail(TLBM_ERR_INVALID,
+                  "report workspace of this stream must grow to %zu bytes, which cannot happen "
+                  "under graph capture: run one report step eagerly first", bytes);
     if (w.ptr) {
       TLBM_HIP(hipDeviceSynchronize());
       TLBM_HIP(hipFree(w.ptr));
@@ -471,6 +477,11 @@ int tlbm_memcpy_async(void* dst, const void* src, size_t bytes, void* stream) {
 int tlbm_memset(void* ptr, int value, size_t bytes, void* stream) {
   if (bytes == 0) return TLBM_OK;
   if (!ptr) return fail(TLBM_ERR_INVALID, "memset of NULL");
+  // memset nodes in captured graphs have given wrong results on replays after the first
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  TLBM_HIP(hipStreamIsCapturing(static_cast<hipStream_t>(stream), &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    return fail(TLBM_ERR_INVALID, "tlbm_memset is not allowed under graph capture");
   TLBM_HIP(hipMemsetAsync(ptr, value, bytes, static_cast<hipStream_t>(stream)));
   return TLBM_OK;
 }
@@ -854,6 +865,39 @@ int tlbm_batch_graph(tlbm_batch* b, void* f_a, void* f_b, int steps, int report_
     return rc;
   }
   if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(e, "hipGraphInstantiate");
+  }
+  tlbm_graph* g = new tlbm_graph();
+  g->graph = graph;
+  g->exec = exec;
+  *out = g;
+  return TLBM_OK;
+}
+
+int tlbm_capture_begin(void* stream) {
+  if (!stream) return fail(TLBM_ERR_INVALID, "graph capture needs a created (non-null) stream");
+  TLBM_HIP(hipStreamBeginCapture(static_cast<hipStream_t>(stream),
+                                 hipStreamCaptureModeThreadLocal));
+  return TLBM_OK;
+}
+
+int tlbm_capture_end(void* stream, tlbm_graph** out) {
+  if (!stream) return fail(TLBM_ERR_INVALID, "graph capture needs a created (non-null) stream");
+  if (out) *out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(static_cast<hipStream_t>(stream), &graph);
+  if (e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return hip_fail(e, "hipStreamEndCapture");
+  }
+  if (!out) {
+    (void)hipGraphDestroy(graph);
+    return fail(TLBM_ERR_INVALID, "out is NULL (the capture was discarded)");
+  }
   hipGraphExec_t exec = nullptr;
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {

@@ -93,6 +93,8 @@ SIGNATURES = {
                                     _vp, _c.POINTER(_vp)]),
     "tlbm_graph_launch": (_c.c_int, [_vp, _vp]),
     "tlbm_graph_destroy": (_c.c_int, [_vp]),
+    "tlbm_capture_begin": (_c.c_int, [_vp]),
+    "tlbm_capture_end": (_c.c_int, [_vp, _c.POINTER(_vp)]),
     "tlbm_device_cus": (_c.c_int, [_c.POINTER(_c.c_int)]),
     "tlbm_step_heat": (_c.c_int, [_c.c_int, _vp, _vp, _vp, _vp, _c.POINTER(Params),
                                   _c.POINTER(Heat), _c.POINTER(Fields), _vp, _vp]),

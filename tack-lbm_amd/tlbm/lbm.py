@@ -679,12 +679,14 @@ class Solver:
 
     def set_launch(self, block_x=0):
         """Threads per block of this solver's quiet and report kernels (see ``set_launch``)."""
+        self._drop_graphs()
         for diag in (self.quiet, self.report):
             set_launch(self.phys, self.geo, diag, self.flow.side, self.storage, block_x,
                        self.heat)
 
     def set_physics(self, collision, reynolds):
         """lbm.py:613-616: nu = u0 D / Re; a collision change selects another kernel."""
+        self._drop_graphs()
         self.collision_name, self.reynolds = collision, float(reynolds)
         self.nu = self.u0 * self.diameter / self.reynolds
         kw = {}
@@ -708,6 +710,7 @@ class Solver:
 
     def set_obstacle(self, kind):
         """lbm.py:628-637."""
+        self._drop_graphs()
         nx, ny, nz, r = self.nx, self.ny, self.nz, self.diameter / 2
         if self._center is None:
             x0, y0, z0 = 0.22 * nx, 0.5 * ny + 1.7, 0.5 * nz
@@ -743,6 +746,59 @@ class Solver:
         self.f.reverse()
         self.g.reverse()
         self.step_count += 1
+
+    def run(self, steps, report_every=0):
+        """``steps`` steps (even), a report step every ``report_every`` of them (0: none), as
+        ``step()`` would take them, replayed from a hipGraph captured on first use for this
+        (steps, report_every).  Needs a created stream (``Solver(stream=...)``).  Graphs hold
+        the plugins' parameters as they were at capture: ``set_physics``, ``set_obstacle`` and
+        ``set_launch`` drop them (``set_actuation`` and wall temperatures are read from memory
+        at every step and need no new capture)."""
+        steps, report_every = int(steps), int(report_every)
+        if self.stream is None:
+            raise ValueError("graph replay needs a created stream (Solver(stream=...))")
+        if steps <= 0 or steps % 2:
+            raise ValueError(f"steps must be even and > 0, got {steps}")
+        if report_every < 0:
+            raise ValueError(f"report_every must be >= 0, got {report_every}")
+        if self.heat_enabled and report_every:
+            raise ValueError("report steps of the temperature variant clear q_solid with a "
+                             "memset, which a graph must not hold: use step()")
+        key = (steps, report_every)
+        graph = self._graphs.get(key)
+        if graph is None:
+            graph = self._graphs[key] = self._capture(steps, report_every)
+        check(lib().tlbm_graph_launch(graph, self.stream), "graph launch")
+        self.step_count += steps
+
+    def _capture(self, steps, report_every):
+        L = lib()
+        order, count = (list(self.f), list(self.g)), self.step_count
+        out = ctypes.c_void_p()
+        check(L.tlbm_capture_begin(self.stream), "capture begin")
+        try:
+            for i in range(steps):
+                self.step(report=report_every > 0 and (i + 1) % report_every == 0)
+        except Exception:
+            if L.tlbm_capture_end(self.stream, ctypes.byref(out)) == 0:
+                L.tlbm_graph_destroy(out)
+            raise
+        finally:
+            self.f, self.g = order
+            self.step_count = count
+        check(L.tlbm_capture_end(self.stream, ctypes.byref(out)), "capture end")
+        return out.value
+
+    def _drop_graphs(self):
+        for graph in getattr(self, "_graphs", {}).values():
+            lib().tlbm_graph_destroy(graph)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self._drop_graphs()
+        except Exception:
+            pass
 
     def monitors(self, interval):
         """lbm.py:649-661, from the FP64 sums of the last report step."""

@@ -299,3 +299,40 @@ def test_step_rejects_bad_arguments():
     with pytest.raises(ValueError):
         tlbm.lbm_step(lat, phys, geo, flow, tlbm.Quiet({k: None for k in tlbm.FIELDS}), a,
                       tlbm.empty((19 * lat.n - 19,)))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_solver_run_graph_replays_match_eager_steps(storage):
+    """Solver.run replays a captured hipGraph: every replay (with eager work and syncs between
+    replays) leaves the state, report fields and FP64 sums that the same steps taken one by one
+    leave; after set_physics the next run captures the new collision."""
+    import ctypes
+    import tlbm
+    L = tlbm._capi.lib()
+    st = ctypes.c_void_p()
+    tlbm._capi.check(L.tlbm_stream_create(ctypes.byref(st)), "stream")
+
+    def make():
+        return tlbm.Solver(96, 48, 32, collision="les", obstacle="sphere", reynolds=400.0,
+                           lateral="lid", noise_amp=1e-3, storage=storage, stream=st.value)
+    g, e = make(), make()
+    for rep in range(4):
+        if rep == 2:
+            g.set_physics("trt", 250.0)
+            e.set_physics("trt", 250.0)
+        g.run(20, report_every=10)
+        for i in range(20):
+            e.step(report=(i + 1) % 10 == 0)
+        assert g.step_count == e.step_count == 20 * (rep + 1)
+        mg, me = g.monitors(10), e.monitors(10)        # eager work + sync between replays
+        assert mg == me, rep
+        assert np.array_equal(g.populations(), e.populations()), rep
+        for k in tlbm.FIELDS:
+            assert np.array_equal(g.fields[k].numpy(), e.fields[k].numpy()), (rep, k)
+    assert len(g._graphs) == 1                       # the TRT graph; the LES one was dropped
+    with pytest.raises(ValueError):
+        g.run(3)
+    with pytest.raises(ValueError):
+        tlbm.Solver(16, 8, 8).run(2)                 # no created stream
+    del g, e
+    L.tlbm_stream_destroy(st)
