@@ -19,11 +19,11 @@ adjoint: the gradient workload (BGK 128x64x64, 2000 steps checkpointed every 100
 backward pass times and the adjoint kernel against the forward step kernel, under "adjoint".
 derived: vorticity magnitude and Q-criterion of the 512x256x256 output grid, under "derived".
 heat: the temperature variant's step against the flow step (heat-sink lattice), under "heat".
-graph: a small lattice stepped one step() at a time and by Solver.run hipGraph replays, under
-"graph".
+graph (with --graph): a small lattice stepped one step() at a time and by Solver.run hipGraph
+replays, under "graph".
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
-                       [--no-adjoint] [--no-derived] [--no-heat] [--no-graph]
+                       [--no-adjoint] [--no-derived] [--no-heat] [--graph]
                        [--storage f32,f16,bf16] [--block N]
                        [--calibrate]
 """
@@ -415,8 +415,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--no-derived", action="store_true", help="skip the derived-field timing")
     ap.add_argument("--no-heat", action="store_true", help="skip the temperature-variant timing")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="skip the small-lattice step-by-step vs hipGraph timing")
+    ap.add_argument("--graph", action="store_true",
+                    help="add the small-lattice step-by-step vs hipGraph timing")
     ap.add_argument("--calibrate", action="store_true",
                     help="also time a plain 1 GiB copy (counter calibration, streaming ceiling)")
     args = ap.parse_args()
@@ -457,7 +457,7 @@ def main():
               f"ms ({heat['heat_over_flow']:.2f}x, {heat['achieved_gbs']:.0f} GB/s)",
               file=sys.stderr, flush=True)
     graph = None
-    if not args.no_graph:
+    if args.graph:
         graph = time_graph()
         print(f"[bench] small lattice (96x48x48): {graph['eager_mlups']:.0f} MLUPS step by step, "
               f"{graph['graph_mlups']:.0f} MLUPS from hipGraph replays", file=sys.stderr,
