@@ -163,3 +163,27 @@ def test_c2_trt_smagorinsky_against_fp64_record(storage):
     print(f"\nC2 {storage}, steps <= {C2_AGREE_STEPS}: max |d drag| / max |drag| {ed:.2e}, "
           f"max |d lift| / max |drag| {el:.2e} (bound {C2_BOUNDS[storage]:.0e})")
     assert ed <= C2_BOUNDS[storage] and el <= C2_BOUNDS[storage]
+
+
+@pytest.mark.parametrize("reynolds", [300.0, 1000.0, 3000.0])
+@pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
+def test_collision_check_stability_matrix(collision, reynolds):
+    """collision_check.py:9-26 at its default 8000 steps: the default Solver at Re 300, 1000 and
+    3000, monitors every 50 steps, stopping at the first non-finite drag.  README.md:65-66: BGK
+    diverges at Re 3000; TRT and LES (Smagorinsky subgrid viscosity) stay stable."""
+    import tlbm
+    s = tlbm.Solver(collision=collision, reynolds=reynolds)
+    blew_up = None
+    while s.step_count < 8000:
+        for _ in range(49):
+            s.step()
+        s.step(report=True)
+        if not np.isfinite(s.monitors(50)["drag"]):
+            blew_up = s.step_count
+            break
+    print(f"\nRe {reynolds:5.0f} {collision}: "
+          + (f"diverged at step {blew_up}" if blew_up else "stable for 8000 steps"))
+    if collision == "bgk" and reynolds == 3000.0:
+        assert blew_up is not None
+    elif collision != "bgk":
+        assert blew_up is None
