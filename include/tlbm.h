@@ -111,7 +111,7 @@ int tlbm_memset(void* ptr, int value, size_t bytes, void* stream);  /* asynchron
 
 /* ---- streams, events (replaces tack.sync, lbm.py:651) ------------------------------------ */
 int tlbm_stream_create(void** stream);
-int tlbm_stream_destroy(void* stream);
+int tlbm_stream_destroy(void* stream);  /* waits for the stream, frees its report workspace */
 int tlbm_stream_sync(void* stream);
 int tlbm_device_sync(void);
 int tlbm_event_create(void** event);

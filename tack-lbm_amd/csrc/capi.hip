@@ -142,6 +142,23 @@ int workspace(size_t bytes, hipStream_t stream, double** out) {
   return TLBM_OK;
 }
 
+// Frees the report workspaces of a stream (on every device) once its work has finished.
+void release_workspaces(hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  int current = 0;
+  const bool restore = hipGetDevice(&current) == hipSuccess;
+  for (auto it = g_ws.begin(); it != g_ws.end();) {
+    if (it->first.second == stream) {
+      if (it->second.ptr && hipSetDevice(it->first.first) == hipSuccess)
+        (void)hipFree(it->second.ptr);
+      it = g_ws.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  if (restore) (void)hipSetDevice(current);
+}
+
 // Sums partials[k][0..nblocks) in a fixed order (block k sums row k): deterministic FP64 totals.
 __global__ void __launch_bounds__(1024) reduce_partials(const double* __restrict__ partials,
                                                         uint32_t nblocks, double* __restrict__ out) {
@@ -495,7 +512,11 @@ int tlbm_stream_create(void** stream) {
 }
 
 int tlbm_stream_destroy(void* stream) {
-  if (stream) TLBM_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  if (!stream) return TLBM_OK;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  TLBM_HIP(hipStreamSynchronize(st));
+  release_workspaces(st);  // the stream's report workspaces, now idle
+  TLBM_HIP(hipStreamDestroy(st));
   return TLBM_OK;
 }
 
