@@ -640,6 +640,14 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
     const float fluid = 1.0f - here;
     float omega_rot = 0.0f;
     if constexpr (Obs::kMoving) omega_rot = a.omega_rot ? *a.omega_rot : 0.0f;
+    // The cell's own populations, bounced back on wall links (lbm.py:427): all 18 loads are
+    // issued together, before any is needed, so a wave near the wall waits for memory once
+    // rather than once per wall link.
+    T own[kQ];
+    static_for<1, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      own[q] = fin[static_cast<uint64_t>(opp(q)) * a.ns + idx];
+    });
     static_for<1, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
@@ -649,7 +657,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
       }
       const float wall = Obs::solid(a, xsrc[1 + cx], ysrc[1 + cy], zsrc[1 + cz]);
       if (wall != 0.0f) {
-        const float b = St::value(fin[static_cast<uint64_t>(o) * a.ns + idx]);
+        const float b = St::value(own[q]);
         float t = 0.0f;
         if constexpr (Obs::kMoving) {
           // wall velocity at the link midpoint (lbm.py:418-425)
