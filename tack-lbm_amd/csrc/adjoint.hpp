@@ -95,18 +95,32 @@ __global__ void __launch_bounds__(256) lbm_adjoint_kernel(const AdjArgs a) {
     return;
   } else {
     // ---- Lam = dJ/df_{t+1}(x): reverse streaming + reverse bounce-back + drag source ----
-    float L[kQ];
+    // wn[r] = phi(x + c_r): the wall fraction of link opp(r) of this cell, w_{opp r}(x).
+    // Where any link has a wall, the cell's own populations (mu^{t+1} and f_t at x) are loaded
+    // together, before they are needed, rather than one round trip per wall link.
+    float L[kQ], wn[kQ];
+    bool walls = false;
     L[0] = a.mu_in[idx];
     static_for<1, kQ>([&](auto qc) {
       constexpr int r = decltype(qc)::value;
       constexpr int cx = kCX[r], cy = kCY[r], cz = kCZ[r];
       const uint32_t yi = at(xm[1 - cx], ym[1 - cy], zm[1 - cz]);  // x + c_r
       L[r] = fl * a.mu_in[static_cast<uint64_t>(r) * a.ns + yi];
-      const float wn = a.phi[yi];  // w_{opp r}(x) = phi(x + c_r)
-      if (wn != 0.0f)
-        L[r] += wn * (a.mu_in[static_cast<uint64_t>(opp(r)) * a.ns + idx] +
-                      2.0f * a.lam_t1 * fl * static_cast<float>(cx));
+      wn[r] = a.phi[yi];
+      walls = walls || wn[r] != 0.0f;
     });
+    if (walls) {
+      float own[kQ];
+      static_for<1, kQ>([&](auto qc) {
+        constexpr int r = decltype(qc)::value;
+        own[r] = a.mu_in[static_cast<uint64_t>(opp(r)) * a.ns + idx];
+      });
+      static_for<1, kQ>([&](auto qc) {
+        constexpr int r = decltype(qc)::value;
+        if (wn[r] != 0.0f)
+          L[r] += wn[r] * (own[r] + 2.0f * a.lam_t1 * fl * static_cast<float>(kCX[r]));
+      });
+    }
 
     // ---- forward recompute of step t at x ----
     float g[kQ];
@@ -115,17 +129,24 @@ __global__ void __launch_bounds__(256) lbm_adjoint_kernel(const AdjArgs a) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
       const uint32_t si = at(xm[1 + cx], ym[1 + cy], zm[1 + cz]);  // x - c_q
-      const float p = a.f_t[static_cast<uint64_t>(q) * a.ns + si];
-      g[q] = p;
-      if constexpr (q > 0) {
-        const float w = a.phi[si];
-        if (w != 0.0f) {
-          const float b = a.f_t[static_cast<uint64_t>(opp(q)) * a.ns + idx];
-          g[q] = p + w * (b - p);
-          dragw += w * b * static_cast<float>(cx);
-        }
-      }
+      g[q] = a.f_t[static_cast<uint64_t>(q) * a.ns + si];
     });
+    if (walls) {
+      float own[kQ];
+      static_for<1, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        own[q] = a.f_t[static_cast<uint64_t>(opp(q)) * a.ns + idx];
+      });
+      static_for<1, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const float w = wn[opp(q)];  // phi(x - c_q)
+        if (w != 0.0f) {
+          const float p = g[q], b = own[q];
+          g[q] = p + w * (b - p);
+          dragw += w * b * static_cast<float>(kCX[q]);
+        }
+      });
+    }
     float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
