@@ -516,16 +516,27 @@ __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z
                     ((a.wrap & 4) && (z == 0 || z == nz - 1));
   const bool walls = wrap || Obs::near(a, x, y, z);
   if (walls) {
+    // the cell's own populations and the facing sinks' temperatures are loaded together,
+    // before any is used (one memory round trip, not one per wall link); t_solid only where a
+    // sink faces the link
+    float own[kQ7], wall[kQ7], hot[kQ7], ts[kQ7];
     static_for<0, kQ7>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = kCX7[q], cy = kCY7[q], cz = kCZ7[q];
-      const float wall = q == 0 ? here : Obs::solid(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]);
-      if (wall != 0.0f) {
-        const float hot = Obs::sink(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]);
-        const float b = gin[static_cast<uint64_t>(opp7(q)) * a.ns + idx];
-        float at_wall = b * (1.0f - 2.0f * hot);
-        if (hot != 0.0f) at_wall += hot * 2.0f * kW7[q] * a.t_solid[nidx[q]];
-        g[q] = g[q] + wall * (at_wall - g[q]);
+      own[q] = gin[static_cast<uint64_t>(opp7(q)) * a.ns + idx];
+      wall[q] = q == 0 ? here : Obs::solid(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]);
+      hot[q] = wall[q] != 0.0f ? Obs::sink(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]) : 0.0f;
+    });
+    static_for<0, kQ7>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      ts[q] = hot[q] != 0.0f ? a.t_solid[nidx[q]] : 0.0f;
+    });
+    static_for<0, kQ7>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if (wall[q] != 0.0f) {
+        float at_wall = own[q] * (1.0f - 2.0f * hot[q]);
+        if (hot[q] != 0.0f) at_wall += hot[q] * 2.0f * kW7[q] * ts[q];
+        g[q] = g[q] + wall[q] * (at_wall - g[q]);
       }
     });
   }
