@@ -117,6 +117,7 @@ struct StepArgs {
   uint64_t ns;               // plane stride (elements)
   int32_t nx, ny, nz;
   int32_t wrap;              // bit 0/1/2: the obstacle reaches across the x/y/z periodic faces
+  uint32_t stream_stores;    // 1: launch the non-temporal-store (NT) quiet kernel (host side)
   float omega;               // BGK
   float tau0, smag, magic;   // smag = 25.456 cs^2 (lbm.py:93)
   float o[8];                // obstacle parameters
@@ -574,7 +575,8 @@ __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z
   }
 }
 
-template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool HEAT = false>
+template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool HEAT = false,
+          bool NT = false>
 __device__ __forceinline__ void step_cell(const StepArgs& a, const typename StorageT<STORAGE>::T* __restrict__ fin,
                                           typename StorageT<STORAGE>::T* __restrict__ fout,
                                           int x, int y, int z, bool active,
@@ -696,9 +698,14 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   CollideExtra ex;
   collide_cell<COLL, kShift, HEAT>(a, f, static_cast<float>(x), here, rho, ux, uy, uz, &ex);
   if (active) {
+    // NT: non-temporal stores, for lattices whose two population buffers are far larger than
+    // the Infinity Cache (measured on MI355X: 1.6% faster at FP32, 3.4% at FP16, 512x256x256);
+    // below that size the next step's loads hit the cache lines plain stores leave there.
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      fout[static_cast<uint64_t>(q) * a.ns + idx] = St::encode(f[q]);
+      T* dst = &fout[static_cast<uint64_t>(q) * a.ns + idx];
+      if constexpr (NT) __builtin_nontemporal_store(St::encode(f[q]), dst);
+      else *dst = St::encode(f[q]);
     });
   }
   if constexpr (HEAT) heat_cell<OBS, REPORT>(a, x, y, z, idx, here, ux, uy, uz, ex, active);
@@ -745,7 +752,8 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
 
 // ---- the step kernel of one lattice -------------------------------------------------------------
 // HEAT: with the temperature step (FP32 populations only).
-template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool HEAT = false>
+template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool HEAT = false,
+          bool NT = false>
 __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   static_assert(!HEAT || STORAGE == TLBM_F32, "the temperature variant stores FP32");
   using T = typename StorageT<STORAGE>::T;
@@ -763,7 +771,7 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   const int z = static_cast<int>(row / static_cast<uint32_t>(a.ny));
   const uint32_t idx = (static_cast<uint32_t>(z) * a.ny + y) * static_cast<uint32_t>(a.nx) + x;
   double acc[TLBM_NSUMS];
-  step_cell<COLL, OBS, SIDE, REPORT, STORAGE, HEAT>(a, static_cast<const T*>(a.fin),
+  step_cell<COLL, OBS, SIDE, REPORT, STORAGE, HEAT, NT>(a, static_cast<const T*>(a.fin),
                                                     static_cast<T*>(a.fout), x, y, z, active,
                                                     a.fld, idx, acc);
   if constexpr (REPORT) {
@@ -819,7 +827,7 @@ __device__ __forceinline__ uint32_t chunk_of_slot(uint32_t slot, uint32_t per) {
   return (slot % kXcds) * per + slot / kXcds;
 }
 
-template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE>
+template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool NT = false>
 __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
   using T = typename StorageT<STORAGE>::T;
   const T* __restrict__ fin = static_cast<const T*>(b.fin);
@@ -838,7 +846,7 @@ __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
     const uint32_t z = div_small(cell, static_cast<uint32_t>(a.nx) * a.ny, D.inv_nxny, yz);
     y = div_small(yz, static_cast<uint32_t>(a.nx), D.inv_nx, x);
     double acc[TLBM_NSUMS];
-    step_cell<COLL, OBS, SIDE, REPORT, STORAGE>(a, fin + D.elem_off, fout + D.elem_off,
+    step_cell<COLL, OBS, SIDE, REPORT, STORAGE, false, NT>(a, fin + D.elem_off, fout + D.elem_off,
                                                 static_cast<int>(x), static_cast<int>(y),
                                                 static_cast<int>(z), active, b.fld,
                                                 D.cell_off + cell, acc);

@@ -15,18 +15,35 @@ using StepTable = StepLaunch[kObstacles][3][2];
 // Launchers of the temperature variant of one collision (FP32), indexed as StepTable.
 using HeatTable = StepLaunch[kObstacles][3][2];
 
+// Quiet flow steps come in two builds: plain and non-temporal population stores (NT), chosen
+// per launch by a.stream_stores (the host sets it for large lattices).  Report and temperature
+// steps always use plain stores.
 template <int C, int O, int S, bool R, int ST, bool H = false>
 hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t stream) {
-  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H>), grid, block, 0, stream, a);
+  if constexpr (!R && !H) {
+    if (a.stream_stores) {
+      hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, true>), grid, block, 0, stream, a);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, false>), grid, block, 0, stream, a);
   return hipGetLastError();
 }
 
-using BatchLaunch = hipError_t (*)(const BatchArgs&, dim3 grid, dim3 block, hipStream_t stream);
+using BatchLaunch = hipError_t (*)(const BatchArgs&, bool nt, dim3 grid, dim3 block,
+                                   hipStream_t stream);
 using BatchTable = BatchLaunch[kObstacles][3][2];
 
+// `nt`: the batch's population buffers are large enough for non-temporal stores (quiet steps).
 template <int C, int O, int S, bool R, int ST>
-hipError_t launch_batch(const BatchArgs& b, dim3 grid, dim3 block, hipStream_t stream) {
-  hipLaunchKernelGGL((lbm_batch_kernel<C, O, S, R, ST>), grid, block, 0, stream, b);
+hipError_t launch_batch(const BatchArgs& b, bool nt, dim3 grid, dim3 block, hipStream_t stream) {
+  if constexpr (!R) {
+    if (nt) {
+      hipLaunchKernelGGL((lbm_batch_kernel<C, O, S, R, ST, true>), grid, block, 0, stream, b);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((lbm_batch_kernel<C, O, S, R, ST, false>), grid, block, 0, stream, b);
   return hipGetLastError();
 }
 
