@@ -633,7 +633,9 @@ int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const
   tlbm::StepArgs a = make_args(*params, d.obstacle);
   a.fin = f_in;
   a.fout = f_out;
-  a.stream_stores = stream_stores(19 * static_cast<uint64_t>(params->qstride), d.storage) ? 1u : 0u;
+  // the temperature step adds 7 FP32 planes (g) to the 19 population planes
+  a.stream_stores = stream_stores((heat ? 26 : 19) * static_cast<uint64_t>(params->qstride),
+                                  heat ? TLBM_F32 : d.storage) ? 1u : 0u;
   if (heat) {
     if (!g_in || !g_out) return fail(TLBM_ERR_INVALID, "g_in or g_out is NULL");
     if (g_in == g_out) return fail(TLBM_ERR_INVALID, "g_in and g_out must be distinct buffers");

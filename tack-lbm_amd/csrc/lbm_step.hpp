@@ -491,7 +491,7 @@ __device__ __forceinline__ void tile_of_block(uint32_t b, uint32_t xblocks, uint
 // anti-bounces to its temperature: g = pulled + wall (b (1 - 2 hot) + 2 hot w t_solid - pulled).
 // BGK relaxation with omega_g = 1 / (4 alpha_eff + 1/2) and the flow's sponge blend.  REPORT:
 // writes the cell temperature and adds fluid * temp to the q_solid of each sink voxel it faces.
-template <int OBS, bool REPORT>
+template <int OBS, bool REPORT, bool NT = false>
 __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z, uint32_t idx,
                                           float here, float ux, float uy, float uz,
                                           const CollideExtra& ex, bool active) {
@@ -554,7 +554,9 @@ __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z
       const float eq = kW7[q] * temp * (1.0f + 4.0f * (cx * ux + cy * uy + cz * uz));
       const float post = g[q] - omega_g * (g[q] - eq);
       const float eqs = kW7[q] * temp_s * (1.0f + 4.0f * (cx * ex.uxs + cy * ex.uys + cz * ex.uzs));
-      a.g_out[static_cast<uint64_t>(q) * a.ns + idx] = eqs + keep * (post - eq);
+      float* dst = &a.g_out[static_cast<uint64_t>(q) * a.ns + idx];
+      if constexpr (NT) __builtin_nontemporal_store(eqs + keep * (post - eq), dst);
+      else *dst = eqs + keep * (post - eq);
     });
   }
   if constexpr (REPORT) {
@@ -708,7 +710,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
       else *dst = St::encode(f[q]);
     });
   }
-  if constexpr (HEAT) heat_cell<OBS, REPORT>(a, x, y, z, idx, here, ux, uy, uz, ex, active);
+  if constexpr (HEAT) heat_cell<OBS, REPORT, NT>(a, x, y, z, idx, here, ux, uy, uz, ex, active);
 
   // ---- report fields and FP64 summands (lbm.py:493-513, 649-661) ----
   if constexpr (REPORT) {

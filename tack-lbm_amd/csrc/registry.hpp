@@ -15,12 +15,12 @@ using StepTable = StepLaunch[kObstacles][3][2];
 // Launchers of the temperature variant of one collision (FP32), indexed as StepTable.
 using HeatTable = StepLaunch[kObstacles][3][2];
 
-// Quiet flow steps come in two builds: plain and non-temporal population stores (NT), chosen
-// per launch by a.stream_stores (the host sets it for large lattices).  Report and temperature
-// steps always use plain stores.
+// Quiet steps come in two builds: plain and non-temporal population stores (NT), chosen per
+// launch by a.stream_stores (the host sets it for large lattices).  Report steps always use
+// plain stores.
 template <int C, int O, int S, bool R, int ST, bool H = false>
 hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t stream) {
-  if constexpr (!R && !H) {
+  if constexpr (!R) {
     if (a.stream_stores) {
       hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, true>), grid, block, 0, stream, a);
       return hipGetLastError();

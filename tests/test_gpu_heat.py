@@ -159,3 +159,33 @@ def test_solver_heat_with_padded_planes():
         out.append((s.populations(), g, s.output["temp"].numpy.copy(), s.q_solid()))
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_large_heat_lattice_repeats_the_small_one():
+    """A temperature step on a lattice large enough for the non-temporal-store kernel (64 x 64 x
+    1024, 26 FP32 planes of 4.2 M cells in each buffer) whose state, solid temperatures and
+    geometry (a z-axis cylinder) repeat every 4 cells along z gives, bit for bit, the step of
+    the 64 x 64 x 4 lattice (plain stores) repeated 256 times."""
+    import tlbm
+    reps = 256
+    small = Case((64, 64, 4), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
+    big = Case((64, 64, 4 * reps), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
+    assert 2 * 26 * 4 * big.n > 512 * 2 ** 20
+    f64 = random_populations(small.n, 41).astype(np.float32).reshape(19, 1, small.n)
+    g64 = _random_g(small.n, 42).astype(np.float32).reshape(7, 1, small.n)
+    ts = (0.5 + np.random.default_rng(43).uniform(size=small.n)).astype(np.float32)
+    out = []
+    for case, r in ((small, 1), (big, reps)):
+        lat, phys, geo, flow = case.device()
+        n = case.n
+        f = [tlbm.array(np.broadcast_to(f64, (19, r, small.n))), tlbm.empty((19 * n,))]
+        g = [tlbm.array(np.broadcast_to(g64, (7, r, small.n))), tlbm.empty((7 * n,))]
+        heat = tlbm.Heat({"t_solid": tlbm.array(np.broadcast_to(ts, (r, small.n))),
+                          "q_solid": tlbm.zeros((n,)), "temp": tlbm.zeros((n,))}, 0.004, 0.006)
+        fields = {k: tlbm.zeros((n,)) for k in tlbm.FIELDS}
+        tlbm.lbm_step(lat, phys, geo, flow, tlbm.Quiet(fields), f[0], f[1], heat, g[0], g[1])
+        tlbm.sync()
+        out.append((f[1].numpy().reshape(19, r, small.n), g[1].numpy().reshape(7, r, small.n)))
+    (sf, sg), (bf, bg) = out
+    assert np.array_equal(bf, np.broadcast_to(sf, bf.shape))
+    assert np.array_equal(bg, np.broadcast_to(sg, bg.shape))

@@ -153,8 +153,10 @@ def test_ragged_and_degenerate_grids(dims):
 def test_planes_beyond_32_bit_offsets(storage):
     """A lattice of 2^27 cells (64 x 64 x 32768): its population planes end beyond 2^31 elements
     and beyond 2^32 bytes.  State, old fields and geometry (a rotating z-axis cylinder) repeat
-    every 4 cells along z, so one report step must give, bit for bit, the step of the 64 x 64 x 4
-    lattice repeated 8192 times, and FP64 sums 8192 times that lattice's."""
+    every 4 cells along z, so a report step and then a quiet step must give, bit for bit, the
+    steps of the 64 x 64 x 4 lattice repeated 8192 times, and FP64 sums 8192 times that
+    lattice's.  The large lattice's quiet step takes the non-temporal-store kernel, the small
+    one's the plain one."""
     import tlbm
     reps = 8192
     small = Case((64, 64, 4), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
@@ -171,13 +173,14 @@ def test_planes_beyond_32_bit_offsets(storage):
         fields = {k: tlbm.array(np.broadcast_to(old[k], (r, small.n))) for k in tlbm.FIELDS}
         sums = tlbm.HostBuffer(14)
         tlbm.lbm_step(lat, phys, geo, flow, tlbm.Report(fields), f_in, f_out, sums=sums)
+        tlbm.lbm_step(lat, phys, geo, flow, tlbm.Quiet(fields), f_out, f_in)
         tlbm.sync()
-        del f_in
-        results.append((f_out, fields, sums.numpy.copy()))
-    (s_out, s_fields, s_sums), (b_out, b_fields, b_sums) = results
-    assert np.array_equal(b_out.numpy().reshape(19, reps, small.n),
-                          np.broadcast_to(s_out.numpy().reshape(19, 1, small.n),
-                                          (19, reps, small.n)))
+        results.append((f_out, f_in, fields, sums.numpy.copy()))
+    (s_out, s_q, s_fields, s_sums), (b_out, b_q, b_fields, b_sums) = results
+    for big_f, small_f in ((b_out, s_out), (b_q, s_q)):
+        assert np.array_equal(big_f.numpy().reshape(19, reps, small.n),
+                              np.broadcast_to(small_f.numpy().reshape(19, 1, small.n),
+                                              (19, reps, small.n)))
     for k in tlbm.FIELDS:
         assert np.array_equal(b_fields[k].numpy().reshape(reps, small.n),
                               np.broadcast_to(s_fields[k].numpy(), (reps, small.n))), k
