@@ -144,6 +144,17 @@ def test_step_argument_errors_are_reported_before_any_launch():
         assert lib.tlbm_set_launch(h, bx) == 0
 
 
+def test_capture_entry_points_reject_the_null_stream():
+    from tlbm import _capi
+    lib = _capi.lib()
+    out = ctypes.c_void_p()
+    assert lib.tlbm_capture_begin(None) != 0
+    assert "stream" in _capi.last_error()
+    assert lib.tlbm_capture_end(None, ctypes.byref(out)) != 0
+    assert out.value is None
+    assert lib.tlbm_graph_destroy(None) == 0
+
+
 def test_shifted_population_codecs():
     """encode/decode of f - w_q: f16 within half an ulp of the deviation, bf16 equal to an
     independent round-to-nearest-even (torch's bfloat16 conversion)."""

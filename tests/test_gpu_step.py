@@ -339,3 +339,35 @@ def test_solver_run_graph_replays_match_eager_steps(storage):
         tlbm.Solver(16, 8, 8).run(2)                 # no created stream
     del g, e
     L.tlbm_stream_destroy(st)
+
+
+def test_capture_refuses_what_a_graph_must_not_hold():
+    """Under capture, tlbm_memset and a report step whose stream has no reduction workspace yet
+    are refused with a message (nothing allocates or records a memset node); the capture still
+    ends cleanly, and the stream works eagerly afterwards."""
+    import ctypes
+    import tlbm
+    L = tlbm._capi.lib()
+    st = ctypes.c_void_p()
+    tlbm._capi.check(L.tlbm_stream_create(ctypes.byref(st)), "stream")
+    case = Case((24, 12, 9), "bgk", "cylinder", "periodic")
+    lat, phys, geo, flow = case.device()
+    f = [tlbm.array(random_populations(case.n, 51).astype(np.float32)), tlbm.empty((19 * case.n,))]
+    buf = tlbm.empty((64,))
+    sums = tlbm.HostBuffer(14)
+    fields = {k: tlbm.zeros((case.n,)) for k in tlbm.FIELDS}
+    out = ctypes.c_void_p()
+    tlbm._capi.check(L.tlbm_capture_begin(st), "capture begin")
+    assert L.tlbm_memset(buf.ptr, 0, 256, st) != 0
+    assert "capture" in tlbm._capi.last_error()
+    with pytest.raises(RuntimeError, match="eagerly"):
+        tlbm.lbm_step(lat, phys, geo, flow, tlbm.Report(fields), f[0], f[1], sums=sums,
+                      stream=st.value)
+    tlbm._capi.check(L.tlbm_capture_end(st, ctypes.byref(out)), "capture end")
+    L.tlbm_graph_destroy(out)
+    # eager use of the same stream: the report step sizes the workspace and runs
+    tlbm.lbm_step(lat, phys, geo, flow, tlbm.Report(fields), f[0], f[1], sums=sums,
+                  stream=st.value)
+    tlbm.sync(st.value)
+    assert np.isfinite(sums.numpy).all() and sums.numpy[13] > 0
+    L.tlbm_stream_destroy(st)
