@@ -52,145 +52,218 @@ __global__ void __launch_bounds__(256)
 // them); BASIS = true, the trilinear cell's gradient at its centre, the mean of the four edge
 // differences along each axis.  Both read the velocity only: no point gradient goes through HBM.
 //
-// Mapping.  A block is 64 x 4 cell columns (one wave per y row) and a chunk of kz cells along z;
-// each thread marches up its column plane by plane.  Per point plane z it forms, for each
-// component, three sums over its 2 x 2 corner points (x in {i, i+1}, y in {j, j+1}):
+// Mapping.  A block is 64 x 4 cell columns (one wave per y row) and a chunk of 32 cells along
+// z; it walks the point planes of its chunk upwards.  Each plane's tile (the block's corner
+// points plus one point before and two after along x and y, 67 x 7, three components) is
+// staged in LDS (double-buffered, one barrier per plane); the global loads of the next plane
+// are issued before the current one is used, so their latency overlaps the arithmetic.  From
+// LDS each thread forms, for each component, three sums over its 2 x 2 corner points (x in
+// {i, i+1}, y in {j, j+1}):
 //   px(z): the x-derivatives (lattice) or the x edge differences (basis),
 //   py(z): the same along y,
 //   s(z):  the four values, whose differences across planes give the z-derivatives;
-// a cell k then takes planes k and k + 1 (lattice z: s at planes k - 1 .. k + 2, one-sided at
-// the grid's faces).  The x and y neighbours come from the cache lines the wave's own loads and
-// the neighbouring rows' loads bring in; blocks go to the XCDs in contiguous runs of tiles so
-// rows that share lines share an L2.
+// cell k takes planes k and k + 1 (lattice z: s at planes k - 1 .. k + 2, one-sided at the
+// grid's faces).  Blocks go to the XCDs in contiguous runs of tiles, so the halo rows that
+// neighbouring blocks share meet in one L2.
 constexpr int kCellTileX = 64;
 constexpr int kCellTileY = 4;
 constexpr int kCellChunkZ = 32;
+constexpr int kHaloX = kCellTileX + 3;   // points x0 - 1 .. x0 + 65 of a tile at x0
+constexpr int kHaloY = kCellTileY + 3;
+constexpr int kPitch = kHaloX + 1;
+constexpr int kTileFloats = 3 * kHaloY * kPitch;
+constexpr int kCellThreads = kCellTileX * kCellTileY;
+constexpr int kSlots = (kHaloY * kHaloX + kCellThreads - 1) / kCellThreads;  // 2
 
+// A thread's corner column in tile coordinates (point (x, y) of the tile is at x - x0 + 1,
+// y - y0 + 1 for a tile of cells starting at x0, y0).
 struct CellColumn {
-  // offsets within a plane of the corner rows' points and of the y neighbour rows
-  uint32_t rj, rj1, ryl, ryh;
-  uint32_t i, xl, xh1;   // x of the corner column i, its left neighbour, i + 1's right neighbour
+  int lx, lxl, lxh1;   // corner column i, its left neighbour, i + 1's right neighbour
+  int ly, lyl, lyh1;   // corner row j, its lower neighbour, j + 1's upper neighbour
   float sx0, sx1, sy0, sy1;
 };
 
+__device__ __forceinline__ float at(const float* t, int c, int ly, int lx) {
+  return t[(c * kHaloY + ly) * kPitch + lx];
+}
+
 template <bool BASIS>
-__device__ __forceinline__ void plane_sums(const float* __restrict__ v, const CellColumn& t,
-                                           float& px, float& py, float& s) {
-  const float b0 = v[t.rj + t.i], c0 = v[t.rj + t.i + 1];
-  const float b1 = v[t.rj1 + t.i], c1 = v[t.rj1 + t.i + 1];
+__device__ __forceinline__ void plane_sums(const float* t, int c, const CellColumn& k, float& px,
+                                           float& py, float& s) {
+  const float b0 = at(t, c, k.ly, k.lx), c0 = at(t, c, k.ly, k.lx + 1);
+  const float b1 = at(t, c, k.ly + 1, k.lx), c1 = at(t, c, k.ly + 1, k.lx + 1);
   s = (b0 + c0) + (b1 + c1);
   if (BASIS) {
     px = (c0 - b0) + (c1 - b1);
     py = (b1 - b0) + (c1 - c0);
   } else {
-    const float a0 = v[t.rj + t.xl], d0 = v[t.rj + t.xh1];
-    const float a1 = v[t.rj1 + t.xl], d1 = v[t.rj1 + t.xh1];
-    const float e0 = v[t.ryl + t.i], e1 = v[t.ryl + t.i + 1];
-    const float h0 = v[t.ryh + t.i], h1 = v[t.ryh + t.i + 1];
+    const float a0 = at(t, c, k.ly, k.lxl), d0 = at(t, c, k.ly, k.lxh1);
+    const float a1 = at(t, c, k.ly + 1, k.lxl), d1 = at(t, c, k.ly + 1, k.lxh1);
+    const float e0 = at(t, c, k.lyl, k.lx), e1 = at(t, c, k.lyl, k.lx + 1);
+    const float h0 = at(t, c, k.lyh1, k.lx), h1 = at(t, c, k.lyh1, k.lx + 1);
     // d/dx at corners (i, y) and (i + 1, y), y = j, j + 1; d/dy at corners (x, j), (x, j + 1)
-    px = ((c0 - a0) + (c1 - a1)) * t.sx0 + ((d0 - b0) + (d1 - b1)) * t.sx1;
-    py = ((b1 - e0) + (c1 - e1)) * t.sy0 + ((h0 - b0) + (h1 - c0)) * t.sy1;
+    px = ((c0 - a0) + (c1 - a1)) * k.sx0 + ((d0 - b0) + (d1 - b1)) * k.sx1;
+    py = ((b1 - e0) + (c1 - e1)) * k.sy0 + ((h0 - b0) + (h1 - c0)) * k.sy1;
   }
 }
 
-__device__ __forceinline__ float corner_sum(const float* __restrict__ v, const CellColumn& t) {
-  return (v[t.rj + t.i] + v[t.rj + t.i + 1]) + (v[t.rj1 + t.i] + v[t.rj1 + t.i + 1]);
-}
-
 template <bool BASIS>
-__global__ void __launch_bounds__(kCellTileX * kCellTileY)
+__global__ void __launch_bounds__(kCellThreads)
     vortex_fields_kernel(const float* __restrict__ ux, const float* __restrict__ uy,
                          const float* __restrict__ uz, float* __restrict__ vorticity,
                          float* __restrict__ qcrit, int mx, int my, int mz, float inv_h,
                          uint32_t xtiles, uint32_t ytiles, uint32_t tiles, uint32_t per_xcd) {
+  __shared__ float lds[2][kTileFloats];
   const uint32_t b = blockIdx.x;
   const uint32_t tile = (b % 8u) * per_xcd + b / 8u;   // XCD b % 8 takes a contiguous run
-  if (tile >= tiles) return;
+  if (tile >= tiles) return;                           // uniform over the block
   const int cx = mx - 1, cy = my - 1, cz = mz - 1;
   const uint32_t tx = tile % xtiles, ty = (tile / xtiles) % ytiles, tz = tile / (xtiles * ytiles);
-  const int i = static_cast<int>(tx) * kCellTileX + static_cast<int>(threadIdx.x % kCellTileX);
-  const int j = static_cast<int>(ty) * kCellTileY + static_cast<int>(threadIdx.x / kCellTileX);
-  if (i >= cx || j >= cy) return;
+  const int x0 = static_cast<int>(tx) * kCellTileX, y0 = static_cast<int>(ty) * kCellTileY;
   const int k0 = static_cast<int>(tz) * kCellChunkZ;
   const int k1 = min(k0 + kCellChunkZ, cz);           // cells k0 .. k1 - 1, corner planes k0 .. k1
-  CellColumn t;
-  const uint32_t umx = static_cast<uint32_t>(mx);
+  // planes zlo .. zhi, walked upwards (lattice: one more on each side for the z-derivatives)
+  const int zlo = (!BASIS && k0 > 0) ? k0 - 1 : k0;
+  const int zhi = BASIS ? k1 : min(k1 + 1, mz - 1);
+  // every thread stages planes and meets the barriers; threads past the grid's last cell
+  // column or row compute the last one's values and store nothing
+  const int ir = x0 + static_cast<int>(threadIdx.x % kCellTileX);
+  const int jr = y0 + static_cast<int>(threadIdx.x / kCellTileX);
+  const bool active = ir < cx && jr < cy;
+  const int i = min(ir, cx - 1), j = min(jr, cy - 1);
+  CellColumn k;
+  const int xl = i > 0 ? i - 1 : 0, xh1 = i + 2 < mx ? i + 2 : i + 1;
   const int yl = j > 0 ? j - 1 : 0, yh1 = j + 2 < my ? j + 2 : j + 1;
-  t.i = static_cast<uint32_t>(i);
-  t.xl = static_cast<uint32_t>(i > 0 ? i - 1 : 0);
-  t.xh1 = static_cast<uint32_t>(i + 2 < mx ? i + 2 : i + 1);
-  t.rj = static_cast<uint32_t>(j) * umx;
-  t.rj1 = t.rj + umx;
-  t.ryl = static_cast<uint32_t>(yl) * umx;
-  t.ryh = static_cast<uint32_t>(yh1) * umx;
-  t.sx0 = inv_h / static_cast<float>(i + 1 - static_cast<int>(t.xl));
-  t.sx1 = inv_h / static_cast<float>(static_cast<int>(t.xh1) - i);
-  t.sy0 = inv_h / static_cast<float>(j + 1 - yl);
-  t.sy1 = inv_h / static_cast<float>(yh1 - j);
+  k.lx = i - x0 + 1;
+  k.lxl = xl - x0 + 1;
+  k.lxh1 = xh1 - x0 + 1;
+  k.ly = j - y0 + 1;
+  k.lyl = yl - y0 + 1;
+  k.lyh1 = yh1 - y0 + 1;
+  k.sx0 = inv_h / static_cast<float>(i + 1 - xl);
+  k.sx1 = inv_h / static_cast<float>(xh1 - i);
+  k.sy0 = inv_h / static_cast<float>(j + 1 - yl);
+  k.sy1 = inv_h / static_cast<float>(yh1 - j);
+  const uint32_t umx = static_cast<uint32_t>(mx);
   const uint32_t plane = umx * static_cast<uint32_t>(my);
   const float* const u[3] = {ux, uy, uz};
-  const float q = BASIS ? 0.25f * inv_h : 0.125f;
 
-  float px[3], py[3], s[3];        // plane kk
-  float ppx[3], ppy[3], pz[3];     // plane kk - 1 (and its z-derivative sum, lattice)
-  float sm[3];                     // s at plane max(kk - 1, 0) (lattice)
+  // Staging: the 67 x 7 tile points of one component are r = threadIdx.x and r + 256 (< 469);
+  // their tile and plane offsets are the same for every component and plane.  Points outside
+  // the grid are never read and not staged (loff < 0).
+  int loff[kSlots];
+  uint32_t goff[kSlots];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) plane_sums<BASIS>(u[c] + k0 * plane, t, px[c], py[c], s[c]);
-  if (!BASIS) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) sm[c] = k0 > 0 ? corner_sum(u[c] + (k0 - 1) * plane, t) : s[c];
+  for (int sl = 0; sl < kSlots; ++sl) {
+    const int r = static_cast<int>(threadIdx.x) + sl * kCellThreads;
+    const int hy = r / kHaloX, hx = r % kHaloX;
+    const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+    const bool in = r < kHaloY * kHaloX && gx >= 0 && gx < mx && gy >= 0 && gy < my;
+    loff[sl] = in ? hy * kPitch + hx : -1;
+    goff[sl] = in ? static_cast<uint32_t>(gy) * umx + static_cast<uint32_t>(gx) : 0u;
   }
-  uint32_t e = static_cast<uint32_t>(i) +
-               static_cast<uint32_t>(cx) * (static_cast<uint32_t>(j) + static_cast<uint32_t>(cy) *
-                                                                        static_cast<uint32_t>(k0));
-  const uint32_t cplane = static_cast<uint32_t>(cx) * static_cast<uint32_t>(cy);
-  for (int kk = k0; kk <= k1; ++kk) {
-    float nx[3], ny[3], ns[3], zs[3];
-    if (kk < k1) {
+  float pre[3][kSlots];   // the next plane's tile points, loaded ahead
+  auto fetch = [&](int z) {
+    const uint32_t zoff = static_cast<uint32_t>(z) * plane;
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-        plane_sums<BASIS>(u[c] + (kk + 1) * plane, t, nx[c], ny[c], ns[c]);
-    } else if (!BASIS) {
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        nx[c] = ny[c] = 0.0f;
-        ns[c] = kk + 1 < mz ? corner_sum(u[c] + (kk + 1) * plane, t) : s[c];
-      }
-    }
-    if (!BASIS) {
-      const float sz = inv_h / static_cast<float>(min(kk + 1, mz - 1) - max(kk - 1, 0));
-#pragma unroll
-      for (int c = 0; c < 3; ++c) zs[c] = (ns[c] - sm[c]) * sz;
-    }
-    if (kk > k0) {
-      float G[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        G[c][0] = q * (ppx[c] + px[c]);
-        G[c][1] = q * (ppy[c] + py[c]);
-        G[c][2] = BASIS ? q * (s[c] - pz[c]) : q * (pz[c] + zs[c]);
-      }
-      if (vorticity) {
-        const float wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
-        vorticity[e] = sqrtf(wx * wx + wy * wy + wz * wz);
-      }
-      if (qcrit) {
-        qcrit[e] = -0.5f * (G[0][0] * G[0][0] + G[1][1] * G[1][1] + G[2][2] * G[2][2]) -
-                   (G[0][1] * G[1][0] + G[0][2] * G[2][0] + G[1][2] * G[2][1]);
-      }
-      e += cplane;
-    }
-    if (kk == k1) break;
+      for (int sl = 0; sl < kSlots; ++sl)
+        pre[c][sl] = loff[sl] >= 0 ? u[c][zoff + goff[sl]] : 0.0f;
+  };
+
+  const float q = BASIS ? 0.25f * inv_h : 0.125f;
+  float s1[3], s2[3], px1[3], px2[3], py1[3], py2[3], jz1[3];   // planes z - 1, z - 2
+  auto emit = [&](int kc, const float (&gx)[3], const float (&gy)[3], const float (&gz)[3]) {
+    if (!active) return;
+    float G[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      ppx[c] = px[c];
-      ppy[c] = py[c];
-      pz[c] = BASIS ? s[c] : zs[c];
-      if (!BASIS) sm[c] = s[c];
-      px[c] = nx[c];
-      py[c] = ny[c];
-      s[c] = ns[c];
+      G[c][0] = q * gx[c];
+      G[c][1] = q * gy[c];
+      G[c][2] = q * gz[c];
     }
+    const uint32_t e = static_cast<uint32_t>(i) +
+                       static_cast<uint32_t>(cx) * (static_cast<uint32_t>(j) +
+                                                    static_cast<uint32_t>(cy) *
+                                                        static_cast<uint32_t>(kc));
+    if (vorticity) {
+      const float wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
+      vorticity[e] = sqrtf(wx * wx + wy * wy + wz * wz);
+    }
+    if (qcrit) {
+      qcrit[e] = -0.5f * (G[0][0] * G[0][0] + G[1][1] * G[1][1] + G[2][2] * G[2][2]) -
+                 (G[0][1] * G[1][0] + G[0][2] * G[2][0] + G[1][2] * G[2][1]);
+    }
+  };
+
+  fetch(zlo);
+  int buf = 0;
+  for (int z = zlo; z <= zhi; ++z) {
+    float* t = lds[buf];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int sl = 0; sl < kSlots; ++sl)
+        if (loff[sl] >= 0) t[c * kHaloY * kPitch + loff[sl]] = pre[c][sl];
+    __syncthreads();
+    buf ^= 1;
+    if (z < zhi) fetch(z + 1);
+    float px[3], py[3], s[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) plane_sums<BASIS>(t, c, k, px[c], py[c], s[c]);
+    if (BASIS) {
+      if (z > k0) {   // cell z - 1 from planes z - 1 and z
+        float gx[3], gy[3], gz[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          gx[c] = px1[c] + px[c];
+          gy[c] = py1[c] + py[c];
+          gz[c] = s[c] - s1[c];
+        }
+        emit(z - 1, gx, gy, gz);
+      }
+    } else {
+      const int cp = z - 1;   // corner plane whose z-derivative plane z completes
+      if (cp >= k0) {
+        const float sz = inv_h / static_cast<float>(z - max(cp - 1, 0));
+        float jz[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) jz[c] = (s[c] - (cp > 0 ? s2[c] : s1[c])) * sz;
+        if (cp > k0) {   // cell cp - 1 from corner planes cp - 1 and cp
+          float gx[3], gy[3], gz[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            gx[c] = px2[c] + px1[c];
+            gy[c] = py2[c] + py1[c];
+            gz[c] = jz1[c] + jz[c];
+          }
+          emit(cp - 1, gx, gy, gz);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) jz1[c] = jz[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      s2[c] = s1[c];
+      s1[c] = s[c];
+      px2[c] = px1[c];
+      px1[c] = px[c];
+      py2[c] = py1[c];
+      py1[c] = py[c];
+    }
+  }
+  if (!BASIS && zhi == k1) {
+    // the grid's last plane is corner plane k1: one-sided z-derivative, then cell k1 - 1
+    float gx[3], gy[3], gz[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      gx[c] = px2[c] + px1[c];
+      gy[c] = py2[c] + py1[c];
+      gz[c] = jz1[c] + (s1[c] - s2[c]) * inv_h;
+    }
+    emit(k1 - 1, gx, gy, gz);
   }
 }
 
