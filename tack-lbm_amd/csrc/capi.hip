@@ -861,9 +861,8 @@ int tlbm_batch_step(tlbm_batch* b, const void* f_in, void* f_out, int report,
     }
     a.partials = sums ? b->d_partials : nullptr;
   }
-  // slots: the chunk count rounded up to a multiple of the 8 XCDs (see chunk_of_slot)
-  const uint32_t slots = (b->nchunks + 7) / 8 * 8;
-  const uint32_t grid = b->grid > 0 ? std::min<uint32_t>(b->grid, slots) : slots;
+  // one block per chunk, or a persistent grid that strides over the chunks
+  const uint32_t grid = b->grid > 0 ? std::min<uint32_t>(b->grid, b->nchunks) : b->nchunks;
   const Decoded& d = b->spec;
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = g_batch_tables[d.storage][d.collision][d.obstacle][d.side][report ? 1 : 0](

@@ -822,22 +822,17 @@ __device__ __forceinline__ uint32_t div_small(uint32_t i, uint32_t d, float inv,
   return q;
 }
 
-// Chunks are handed to XCDs in contiguous runs: XCD j (blocks j, j + 8, ...) takes chunks
-// [j * per, (j + 1) * per) in order, so consecutive chunks of a domain, which share a cache line
-// at their edge (the x +- 1 pulls), meet in the same L2.
-__device__ __forceinline__ uint32_t chunk_of_slot(uint32_t slot, uint32_t per) {
-  return (slot % kXcds) * per + slot / kXcds;
-}
-
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool NT = false>
 __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
   using T = typename StorageT<STORAGE>::T;
   const T* __restrict__ fin = static_cast<const T*>(b.fin);
   T* __restrict__ fout = static_cast<T*>(b.fout);
-  const uint32_t per = (b.nchunks + kXcds - 1) / kXcds;
-  for (uint32_t slot = blockIdx.x; slot < per * kXcds; slot += gridDim.x) {
-    const uint32_t c = chunk_of_slot(slot, per);
-    if (c >= b.nchunks) continue;  // uniform: the whole block skips the empty slot
+  // Chunks in plain order: consecutive chunks (neighbouring rows) go to consecutive XCDs, so
+  // the eight XCDs stream through nearby addresses of every plane together.  Handing each XCD
+  // a contiguous eighth of the chunks instead (to keep the x +- 1 edge lines of consecutive
+  // chunks in one L2) measured 7-9% slower at FP32 on MI355X: eight streams per plane, each
+  // an eighth of the lattice apart, contend for the same HBM channels.
+  for (uint32_t c = blockIdx.x; c < b.nchunks; c += gridDim.x) {
     const BatchDomain& D = b.dom[b.chunk_dom[c]];
     const StepArgs& a = D.a;
     const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * a.nz;
