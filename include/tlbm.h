@@ -55,7 +55,7 @@ enum { TLBM_F32 = 0, TLBM_F16 = 1, TLBM_BF16 = 2 };
 
 typedef struct {
   int32_t collision; /* TLBM_BGK / TLBM_TRT / TLBM_LES           */
-  int32_t obstacle;  /* TLBM_CYLINDER / TLBM_SPHERE / TLBM_BOX / TLBM_FIELD */
+  int32_t obstacle;  /* TLBM_CYLINDER / TLBM_SPHERE / TLBM_BOX / TLBM_FIELD / TLBM_HEATSINK */
   int32_t diag;      /* TLBM_QUIET / TLBM_REPORT                  */
   int32_t side;      /* TLBM_PERIODIC_Y / TLBM_SLIP_Y / TLBM_LID_Y */
   int32_t storage;   /* TLBM_F32 / TLBM_F16 / TLBM_BF16           */
