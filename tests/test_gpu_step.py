@@ -371,3 +371,27 @@ def test_capture_refuses_what_a_graph_must_not_hold():
     tlbm.sync(st.value)
     assert np.isfinite(sums.numpy).all() and sums.numpy[13] > 0
     L.tlbm_stream_destroy(st)
+
+
+def test_solver_autotune_keeps_state_and_caches(tmp_path, monkeypatch):
+    """tune_solver times each block size on the solver's own buffers without changing its
+    state, applies the fastest, and answers the second call from its cache; it refuses a
+    solver with the temperature variant."""
+    import tlbm
+    from tlbm import autotune
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "tune.json"))
+    s = tlbm.Solver(200, 40, 24, collision="les", obstacle="sphere", reynolds=400.0,
+                    diameter=10.0, noise_amp=1e-3)
+    before = s.populations()
+    block, ms = autotune.tune_solver(s, reps=2)
+    assert block in (64, 128, 192, 256) and ms > 0
+    assert np.array_equal(s.populations(), before)
+    assert autotune.tune_solver(s, reps=2)[0] == block
+    s.step()                                  # the tuned launch steps as the default does
+    ref = tlbm.Solver(200, 40, 24, collision="les", obstacle="sphere", reynolds=400.0,
+                      diameter=10.0, noise_amp=1e-3)
+    ref.step()
+    assert np.array_equal(s.populations(), ref.populations())
+    h = tlbm.Solver(48, 24, 16, obstacle="heatsink", heat=True, diameter=6.0)
+    with pytest.raises(NotImplementedError):
+        autotune.tune_solver(h)
