@@ -173,3 +173,33 @@ def test_sweep_domains_are_seeded_and_in_range():
     for d in a:
         assert 64 <= d.nx <= 192 and d.ny == d.nz == d.nx // 2
         assert 100 <= d.reynolds <= 1000 and 8 <= d.diameter <= 24
+
+
+def test_batch_domain_near_the_cell_limit_steps_as_the_single_lattice():
+    """A batch of one 256 x 256 x 255 domain (16,711,680 cells, just under the 2^24 a batched
+    domain may hold: the largest cell indices for the float-reciprocal index split) plus a
+    small one steps as the single-lattice kernel (LES, FP32, from the same seeded initial
+    state): within 2 ulp, the bar of the batch parity test (the two kernels contract the
+    sponge blend of q = 0 into FMAs differently), and bit for bit outside the sponge."""
+    import tlbm
+    from tlbm.batch import BatchSolver, Domain
+    big = Domain(256, 256, 255, 3000.0, 32.0, center=(96.0, 128.0, 127.5))
+    b = BatchSolver([Domain(20, 10, 10, 150.0, 5.0), big], collision="les", obstacle="sphere",
+                    lateral="lid", storage="f32", noise_amp=1e-3)
+    b.step()
+    tlbm.sync()
+    got = _view(b.f[0].numpy(), b, 1)
+    s = tlbm.Solver(256, 256, 255, collision="les", obstacle="sphere", reynolds=3000.0,
+                    diameter=32.0, center=(96.0, 128.0, 127.5), lateral="lid", noise_amp=1e-3,
+                    qstride=b.plane_strides[1])
+    # the Solver takes one report step at construction (lbm.py:583-584); restart it from the
+    # batch's initial state
+    tlbm.lbm_init(s.lat, s.geo, s.flow, s.f[0], 0.05, 1e-3, 0)
+    s.step()
+    tlbm.sync()
+    ref = s.populations()
+    assert np.abs(got - ref).max() <= 2 * 2.0 ** -23 * np.abs(ref).max()
+    x = np.arange(big.n) % big.nx
+    fluid_side = x < 0.75 * big.nx                    # upstream of the sponge
+    assert np.array_equal(got[:, fluid_side], ref[:, fluid_side])
+    b.close()
