@@ -276,6 +276,16 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
 // for a row (ceil(nx / b) * b), the larger on a tie.  Measured on MI355X over rows of 96 to 1024
 // cells (LES, FP32 and FP16), this picks the fastest block or one within 2% of it; 256 on rows
 // it does not divide costs up to 25% (384 cells: a second block half idle).
+// The step kernel's cell mapping: the row mapping (blocks along x-rows, y and z uniform per
+// block) where rows are a multiple of 64 cells, else the linear one (blocks of consecutive
+// cells), so that every wave stores whole aligned cache lines.  Measured on MI355X at 33.5 M
+// cells, rows of 500 cells took 1.347 ms (FP32) and 0.707 ms (FP16) per step with the row
+// mapping against 0.897 and 0.462 ms for rows of 512.  The linear mapping's index split
+// (div_small) needs nx + 256 and ny + 256 below 2^24.
+bool linear_mapping(int nx, int ny) {
+  return nx % 64 != 0 && nx <= (1 << 24) - 256 && ny <= (1 << 24) - 256;
+}
+
 int default_block_x(int nx, int) {
   int best = 256;
   int64_t lanes = (static_cast<int64_t>(nx) + 255) / 256 * 256;
@@ -653,16 +663,27 @@ int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const
     a.nu_mol = heat->nu;
     a.pr_t = heat->pr_t;
   }
-  const int bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(params->nx, d.storage);
-  const uint64_t xblocks = (params->nx + bx - 1) / bx;
   const uint64_t rows = static_cast<uint64_t>(params->ny) * params->nz;
-  if (xblocks * rows * bx > 0xFFFFFFFFull)
+  uint64_t nblocks64;
+  int bx;
+  if (linear_mapping(params->nx, params->ny)) {
+    bx = g_block_x[handle] ? g_block_x[handle] : 256;
+    nblocks64 = (rows * params->nx + bx - 1) / bx;
+    a.linear = 1;
+    a.inv_nx = 1.0f / static_cast<float>(params->nx);
+    a.inv_ny = 1.0f / static_cast<float>(params->ny);
+  } else {
+    bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(params->nx, d.storage);
+    const uint64_t xblocks = (params->nx + bx - 1) / bx;
+    nblocks64 = xblocks * rows;
+    a.xblocks = static_cast<uint32_t>(xblocks);
+  }
+  if (nblocks64 * bx > 0xFFFFFFFFull)
     return fail(TLBM_ERR_INVALID, "grid of %llu threads exceeds the launch limit",
-                static_cast<unsigned long long>(xblocks * rows * bx));
-  a.xblocks = static_cast<uint32_t>(xblocks);
+                static_cast<unsigned long long>(nblocks64 * bx));
   a.rows = static_cast<uint32_t>(rows);
   const dim3 block(bx);
-  const dim3 grid(static_cast<uint32_t>(xblocks * rows));
+  const dim3 grid(static_cast<uint32_t>(nblocks64));
   const uint32_t nblocks = grid.x * grid.y * grid.z;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (d.diag == TLBM_REPORT) {

@@ -1,8 +1,10 @@
 // Device code of the fused D3Q19 step for gfx950.
 //
-// Mapping.  One thread per cell; each block covers part of one x-row (tile_of_block maps block
-// ids to rows XCD by XCD), so every y/z decision (periodic wrap, slip mirror, moving lid) is
-// uniform across the wave and every row/plane offset is a scalar.  Each population plane is read and written in coalesced
+// Mapping.  One thread per cell.  Where rows are a multiple of 64 cells each block covers part
+// of one x-row (tile_of_block maps block ids to rows XCD by XCD), so every y/z decision
+// (periodic wrap, slip mirror, moving lid) is uniform across the wave and every row/plane
+// offset is a scalar; other lattices take the linear mapping (blocks of consecutive cells, see
+// lbm_step_kernel), whose waves stay on aligned cache lines.  Each population plane is read and written in coalesced
 // wave-wide runs; the +-1 x offsets of the pull scheme are plain misaligned dword loads, served
 // by the same cache lines.  Measured on MI355X, this form streams as fast as 8/16-byte per-lane
 // vector loads with lane shuffles or two cells per thread, and runs the 16-bit storage formats
@@ -131,6 +133,8 @@ struct StepArgs {
   double* partials;          // report: per-block FP64 partial sums [TLBM_NSUMS][nblocks]
   uint32_t nblocks;
   uint32_t xblocks, rows;    // launch shape: blocks per x-row, ny * nz rows
+  uint32_t linear;           // 1: launch the linear-mapping kernel (host side)
+  float inv_nx, inv_ny;      // 1/nx, 1/ny for the linear mapping's index split
   // temperature variant (lbm.py:457-492)
   const float* g_in;
   float* g_out;
@@ -752,26 +756,76 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   }
 }
 
+// q = i / d, r = i % d for i < 2^24, with a float reciprocal and one correction each way.
+__device__ __forceinline__ uint32_t div_small(uint32_t i, uint32_t d, float inv, uint32_t& r) {
+  uint32_t q = static_cast<uint32_t>(static_cast<float>(i) * inv);
+  int32_t rr = static_cast<int32_t>(i) - static_cast<int32_t>(q * d);
+  if (rr < 0) {
+    q -= 1;
+    rr += static_cast<int32_t>(d);
+  }
+  if (rr >= static_cast<int32_t>(d)) {
+    q += 1;
+    rr -= static_cast<int32_t>(d);
+  }
+  r = static_cast<uint32_t>(rr);
+  return q;
+}
+
 // ---- the step kernel of one lattice -------------------------------------------------------------
 // HEAT: with the temperature step (FP32 populations only).
+// LIN: the linear mapping.  Block b takes cells [b * B, b * B + B) in index order, so every
+// wave reads and writes whole aligned cache lines of each plane whatever nx is; y and z are
+// per lane.  The row mapping (one block per part of an x-row) starts each row's waves at the
+// row's first cell: when nx is not a multiple of 64 those runs straddle cache lines that the
+// neighbouring rows, on other XCDs, write too, which measured 1.5x slower on MI355X.  The host
+// takes LIN for such rows.
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool HEAT = false,
-          bool NT = false>
+          bool NT = false, bool LIN = false>
 __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   static_assert(!HEAT || STORAGE == TLBM_F32, "the temperature variant stores FP32");
   using T = typename StorageT<STORAGE>::T;
-  uint32_t xb, row;
-  tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
-  const int xr = static_cast<int>(xb * blockDim.x + threadIdx.x);
-  const bool active = xr < a.nx;
-  if constexpr (!REPORT) {
-    if (!active) return;
+  int x, y, z;
+  uint32_t idx;
+  bool active;
+  if constexpr (LIN) {
+    const uint32_t nx = static_cast<uint32_t>(a.nx), ny = static_cast<uint32_t>(a.ny);
+    const uint32_t n = nx * ny * static_cast<uint32_t>(a.nz);
+    // block-uniform start: row0 = z0 * ny + y0, column x0; lane t lies (x0 + t) / nx rows on
+    // (x0 + t < nx + 256 and y0 + rows < ny + 256: div_small's range, checked on the host)
+    const uint32_t c0 = blockIdx.x * blockDim.x;
+    const uint32_t row0 = c0 / nx;
+    const uint32_t x0 = c0 - row0 * nx;
+    const uint32_t z0 = row0 / ny;
+    const uint32_t y0 = row0 - z0 * ny;
+    const uint32_t cell = c0 + threadIdx.x;
+    active = cell < n;
+    if constexpr (!REPORT) {
+      if (!active) return;
+    }
+    uint32_t xr, yr;
+    const uint32_t drow = div_small(x0 + threadIdx.x, nx, a.inv_nx, xr);
+    const uint32_t dz = div_small(y0 + drow, ny, a.inv_ny, yr);
+    // report variant: lanes past the last cell mirror it (in-bounds loads) and store nothing
+    x = active ? static_cast<int>(xr) : a.nx - 1;
+    y = active ? static_cast<int>(yr) : a.ny - 1;
+    z = active ? static_cast<int>(z0 + dz) : a.nz - 1;
+    idx = active ? cell : n - 1;
+  } else {
+    uint32_t xb, row;
+    tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
+    const int xr = static_cast<int>(xb * blockDim.x + threadIdx.x);
+    active = xr < a.nx;
+    if constexpr (!REPORT) {
+      if (!active) return;
+    }
+    // in the report variant every lane reaches the block reduction: lanes past the end of the
+    // row mirror the last cell (in-bounds loads) and store nothing
+    x = active ? xr : a.nx - 1;
+    y = static_cast<int>(row % static_cast<uint32_t>(a.ny));
+    z = static_cast<int>(row / static_cast<uint32_t>(a.ny));
+    idx = (static_cast<uint32_t>(z) * a.ny + y) * static_cast<uint32_t>(a.nx) + x;
   }
-  // in the report variant every lane reaches the block reduction: lanes past the end of the
-  // row mirror the last cell (in-bounds loads) and store nothing
-  const int x = active ? xr : a.nx - 1;
-  const int y = static_cast<int>(row % static_cast<uint32_t>(a.ny));
-  const int z = static_cast<int>(row / static_cast<uint32_t>(a.ny));
-  const uint32_t idx = (static_cast<uint32_t>(z) * a.ny + y) * static_cast<uint32_t>(a.nx) + x;
   double acc[TLBM_NSUMS];
   step_cell<COLL, OBS, SIDE, REPORT, STORAGE, HEAT, NT>(a, static_cast<const T*>(a.fin),
                                                     static_cast<T*>(a.fout), x, y, z, active,
@@ -805,22 +859,6 @@ struct BatchArgs {
   tlbm_fields fld;   // packed over all domains (cell_off), or all null
   double* partials;  // report: [TLBM_NSUMS][nchunks]
 };
-
-// q = i / d, r = i % d for i < 2^24, with a float reciprocal and one correction each way.
-__device__ __forceinline__ uint32_t div_small(uint32_t i, uint32_t d, float inv, uint32_t& r) {
-  uint32_t q = static_cast<uint32_t>(static_cast<float>(i) * inv);
-  int32_t rr = static_cast<int32_t>(i) - static_cast<int32_t>(q * d);
-  if (rr < 0) {
-    q -= 1;
-    rr += static_cast<int32_t>(d);
-  }
-  if (rr >= static_cast<int32_t>(d)) {
-    q += 1;
-    rr -= static_cast<int32_t>(d);
-  }
-  r = static_cast<uint32_t>(rr);
-  return q;
-}
 
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool NT = false>
 __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {

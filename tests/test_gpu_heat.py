@@ -68,10 +68,12 @@ COMBOS = [(c, o, s) for c in ("bgk", "trt", "les")
           for s in ("periodic", "slip", "lid")]
 
 
+# rows of 24 cells take the linear cell mapping, rows of 64 the row mapping
+@pytest.mark.parametrize("nx", [24, 64])
 @pytest.mark.parametrize("report", [False, True])
 @pytest.mark.parametrize("collision,obstacle,side", COMBOS)
-def test_heat_step_matches_oracle(collision, obstacle, side, report):
-    case = Case(DIMS, collision, obstacle, side)
+def test_heat_step_matches_oracle(collision, obstacle, side, report, nx):
+    case = Case((nx,) + DIMS[1:], collision, obstacle, side)
     n = case.n
     f = random_populations(n, seed=21)
     g = _random_g(n, seed=22)
@@ -161,15 +163,17 @@ def test_solver_heat_with_padded_planes():
         assert np.array_equal(a, b)
 
 
-def test_large_heat_lattice_repeats_the_small_one():
-    """A temperature step on a lattice large enough for the non-temporal-store kernel (64 x 64 x
-    1024, 26 FP32 planes of 4.2 M cells in each buffer) whose state, solid temperatures and
+@pytest.mark.parametrize("nx", [64, 60])
+def test_large_heat_lattice_repeats_the_small_one(nx):
+    """A temperature step on a lattice large enough for the non-temporal-store kernel (nx x 64 x
+    1024, 26 FP32 planes of about 4 M cells in each buffer) whose state, solid temperatures and
     geometry (a z-axis cylinder) repeat every 4 cells along z gives, bit for bit, the step of
-    the 64 x 64 x 4 lattice (plain stores) repeated 256 times."""
+    the nx x 64 x 4 lattice (plain stores) repeated 256 times.  nx = 64: the row mapping;
+    nx = 60: the linear one."""
     import tlbm
     reps = 256
-    small = Case((64, 64, 4), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
-    big = Case((64, 64, 4 * reps), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
+    small = Case((nx, 64, 4), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
+    big = Case((nx, 64, 4 * reps), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
     assert 2 * 26 * 4 * big.n > 512 * 2 ** 20
     f64 = random_populations(small.n, 41).astype(np.float32).reshape(19, 1, small.n)
     g64 = _random_g(small.n, 42).astype(np.float32).reshape(7, 1, small.n)

@@ -83,10 +83,15 @@ COMBOS = [(c, o, s) for c in ("bgk", "trt", "les")
           for s in ("periodic", "slip", "lid")]
 
 
+# Rows of 24 cells take the linear cell mapping, rows of 64 the row mapping.
+MAPPING_NX = [24, 64]
+
+
+@pytest.mark.parametrize("nx", MAPPING_NX)
 @pytest.mark.parametrize("block_x", [0, 64])
 @pytest.mark.parametrize("collision,obstacle,side", COMBOS)
-def test_quiet_step_f32(collision, obstacle, side, block_x):
-    case = Case((24, 12, 9), collision, obstacle, side)
+def test_quiet_step_f32(collision, obstacle, side, block_x, nx):
+    case = Case((nx, 12, 9), collision, obstacle, side)
     f = random_populations(case.n, seed=11)
     got, ref, *_ = _run_step(case, f, report=False, block_x=block_x)
     assert _rel(got, ref) < 1e-6
@@ -112,10 +117,11 @@ def test_rows_across_blocks(storage, dims, side, block_x):
         assert bool(np.all(np.abs(got - ref) <= bound))
 
 
+@pytest.mark.parametrize("nx", MAPPING_NX)
 @pytest.mark.parametrize("block_x", [0, 64])
 @pytest.mark.parametrize("collision,obstacle,side", COMBOS)
-def test_report_step_f32(collision, obstacle, side, block_x):
-    case = Case((24, 12, 9), collision, obstacle, side)
+def test_report_step_f32(collision, obstacle, side, block_x, nx):
+    case = Case((nx, 12, 9), collision, obstacle, side)
     f = random_populations(case.n, seed=12)
     got, ref, g, o, sums = _run_step(case, f, report=True, block_x=block_x)
     assert _rel(got, ref) < 1e-6
@@ -149,18 +155,20 @@ def test_ragged_and_degenerate_grids(dims):
     assert _rel(got, ref) < 1e-6
 
 
+@pytest.mark.parametrize("nx", [64, 60])
 @pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_planes_beyond_32_bit_offsets(storage):
-    """A lattice of 2^27 cells (64 x 64 x 32768): its population planes end beyond 2^31 elements
-    and beyond 2^32 bytes.  State, old fields and geometry (a rotating z-axis cylinder) repeat
-    every 4 cells along z, so a report step and then a quiet step must give, bit for bit, the
-    steps of the 64 x 64 x 4 lattice repeated 8192 times, and FP64 sums 8192 times that
-    lattice's.  The large lattice's quiet step takes the non-temporal-store kernel, the small
-    one's the plain one."""
+def test_planes_beyond_32_bit_offsets(storage, nx):
+    """A lattice of 2^27 cells (64 x 64 x 32768; with nx = 60, 15/16 of that): its population
+    planes end beyond 2^31 elements and beyond 2^32 bytes.  State, old fields and geometry (a
+    rotating z-axis cylinder) repeat every 4 cells along z, so a report step and then a quiet
+    step must give, bit for bit, the steps of the nx x 64 x 4 lattice repeated 8192 times, and
+    FP64 sums 8192 times that lattice's.  The large lattice's quiet step takes the
+    non-temporal-store kernel, the small one's the plain one; nx = 64 runs the row mapping,
+    nx = 60 the linear one (blocks of 256 cells straddle rows and planes)."""
     import tlbm
     reps = 8192
-    small = Case((64, 64, 4), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
-    big = Case((64, 64, 4 * reps), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
+    small = Case((nx, 64, 4), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
+    big = Case((nx, 64, 4 * reps), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
     assert 19 * big.n > 2 ** 31
     raw = tlbm.encode_populations(random_populations(small.n, seed=31).astype(np.float32),
                                   storage).reshape(19, 1, small.n)
@@ -229,7 +237,8 @@ def test_report_steps_on_two_streams_keep_their_own_sums():
 @pytest.mark.parametrize("storage,bits", [("f16", 10), ("bf16", 7)])
 @pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
 @pytest.mark.parametrize("side", ["periodic", "slip", "lid"])
-@pytest.mark.parametrize("nx,obstacle", [(24, "cylinder"), (130, "cylinder"), (24, "field")])
+@pytest.mark.parametrize("nx,obstacle", [(24, "cylinder"), (64, "cylinder"), (130, "cylinder"),
+                                         (24, "field")])
 def test_shifted_storage(storage, bits, collision, side, nx, obstacle):
     case = Case((nx, 12, 9), collision, obstacle, side)
     f = random_populations(case.n, seed=15)
