@@ -44,20 +44,32 @@ struct AdjArgs {
   int32_t blk[3];        // parameter block origin
   int32_t bn[3];         // parameter block size
   double* gphi;          // dJ/dphi over the parameter block, x fastest
-  uint32_t xblocks, rows;
+  uint32_t xblocks, rows;  // row mapping
+  float inv_nx, inv_ny;    // linear mapping (LIN), as in lbm_step_kernel
 };
 
-template <bool FINAL>
+template <bool FINAL, bool LIN>
 __global__ void __launch_bounds__(256) lbm_adjoint_kernel(const AdjArgs a) {
-  uint32_t xb, row;
-  tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
-  const int x = static_cast<int>(xb * blockDim.x + threadIdx.x);
-  if (x >= a.nx) return;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const int y = static_cast<int>(row % static_cast<uint32_t>(ny));
-  const int z = static_cast<int>(row / static_cast<uint32_t>(ny));
   const uint32_t snx = static_cast<uint32_t>(nx), plane = snx * static_cast<uint32_t>(ny);
-  const uint32_t idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
+  int x, y, z;
+  uint32_t idx;
+  if constexpr (LIN) {
+    uint32_t xr, yr, zr;
+    idx = linear_cell(snx, ny, a.inv_nx, a.inv_ny, xr, yr, zr);
+    if (idx >= plane * static_cast<uint32_t>(nz)) return;
+    x = static_cast<int>(xr);
+    y = static_cast<int>(yr);
+    z = static_cast<int>(zr);
+  } else {
+    uint32_t xb, row;
+    tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
+    x = static_cast<int>(xb * blockDim.x + threadIdx.x);
+    if (x >= nx) return;
+    y = static_cast<int>(row % static_cast<uint32_t>(ny));
+    z = static_cast<int>(row / static_cast<uint32_t>(ny));
+    idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
+  }
   // neighbour coordinates, indexed [1 + c]: x - c (upstream) ...
   const int xm[3] = {x == nx - 1 ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};
   const int ym[3] = {y == ny - 1 ? 0 : y + 1, y, y == 0 ? ny - 1 : y - 1};

@@ -1019,20 +1019,29 @@ int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* 
                   block_origin[i], block_origin[i] + block_size[i], dims[i], i);
   }
   a.gphi = gphi;
-  // rows of up to 128 cells take blocks of 64/128 threads, as the forward step does: a 256-thread
-  // block on a 128-cell row would leave half its waves idle
-  const int bx = default_block_x(params->nx, TLBM_F32);
+  // the cell mapping of the forward step: rows that are a multiple of 64 cells take the row
+  // mapping, with blocks of 64/128 threads on rows of up to 128 cells (a 256-thread block on a
+  // 128-cell row would leave half its waves idle); other rows the linear mapping
+  const bool linear = linear_mapping(params->nx, params->ny);
+  const uint64_t rows = static_cast<uint64_t>(params->ny) * params->nz;
+  const int bx = linear ? 256 : default_block_x(params->nx, TLBM_F32);
   const uint64_t xblocks = (params->nx + bx - 1) / bx;
+  const uint64_t nblocks = linear ? (rows * params->nx + bx - 1) / bx : xblocks * rows;
   a.xblocks = static_cast<uint32_t>(xblocks);
-  a.rows = static_cast<uint32_t>(static_cast<uint64_t>(params->ny) * params->nz);
-  if (xblocks * a.rows * bx > 0xFFFFFFFFull)
-    return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
-  const dim3 grid(static_cast<uint32_t>(xblocks * a.rows)), block(bx);
+  a.rows = static_cast<uint32_t>(rows);
+  a.inv_nx = 1.0f / static_cast<float>(params->nx);
+  a.inv_ny = 1.0f / static_cast<float>(params->ny);
+  if (nblocks * bx > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
+  const dim3 grid(static_cast<uint32_t>(nblocks)), block(bx);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (final_links)
-    hipLaunchKernelGGL(tlbm::lbm_adjoint_kernel<true>, grid, block, 0, st, a);
+  if (final_links && linear)
+    hipLaunchKernelGGL((tlbm::lbm_adjoint_kernel<true, true>), grid, block, 0, st, a);
+  else if (final_links)
+    hipLaunchKernelGGL((tlbm::lbm_adjoint_kernel<true, false>), grid, block, 0, st, a);
+  else if (linear)
+    hipLaunchKernelGGL((tlbm::lbm_adjoint_kernel<false, true>), grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL(tlbm::lbm_adjoint_kernel<false>, grid, block, 0, st, a);
+    hipLaunchKernelGGL((tlbm::lbm_adjoint_kernel<false, false>), grid, block, 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "adjoint launch");
   return TLBM_OK;

@@ -772,6 +772,24 @@ __device__ __forceinline__ uint32_t div_small(uint32_t i, uint32_t d, float inv,
   return q;
 }
 
+// The linear mapping's cell of this lane: block b takes cells [b * B, b * B + B) in index order.
+// The block's first row (row0 = z0 * ny + y0, column x0) is block-uniform; lane t lies
+// (x0 + t) / nx rows on.  Returns the cell index with its (x, y, z); lanes past the last cell
+// get coordinates beyond the lattice, so the caller tests the index against the cell count.
+// div_small's range needs nx + 256 and ny + 256 below 2^24 (checked on the host).
+__device__ __forceinline__ uint32_t linear_cell(uint32_t nx, uint32_t ny, float inv_nx,
+                                                float inv_ny, uint32_t& x, uint32_t& y,
+                                                uint32_t& z) {
+  const uint32_t c0 = blockIdx.x * blockDim.x;
+  const uint32_t row0 = c0 / nx;
+  const uint32_t x0 = c0 - row0 * nx;
+  const uint32_t z0 = row0 / ny;
+  const uint32_t y0 = row0 - z0 * ny;
+  const uint32_t drow = div_small(x0 + threadIdx.x, nx, inv_nx, x);
+  z = z0 + div_small(y0 + drow, ny, inv_ny, y);
+  return c0 + threadIdx.x;
+}
+
 // ---- the step kernel of one lattice -------------------------------------------------------------
 // HEAT: with the temperature step (FP32 populations only).
 // LIN: the linear mapping.  Block b takes cells [b * B, b * B + B) in index order, so every
@@ -789,27 +807,17 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   uint32_t idx;
   bool active;
   if constexpr (LIN) {
-    const uint32_t nx = static_cast<uint32_t>(a.nx), ny = static_cast<uint32_t>(a.ny);
-    const uint32_t n = nx * ny * static_cast<uint32_t>(a.nz);
-    // block-uniform start: row0 = z0 * ny + y0, column x0; lane t lies (x0 + t) / nx rows on
-    // (x0 + t < nx + 256 and y0 + rows < ny + 256: div_small's range, checked on the host)
-    const uint32_t c0 = blockIdx.x * blockDim.x;
-    const uint32_t row0 = c0 / nx;
-    const uint32_t x0 = c0 - row0 * nx;
-    const uint32_t z0 = row0 / ny;
-    const uint32_t y0 = row0 - z0 * ny;
-    const uint32_t cell = c0 + threadIdx.x;
+    const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * static_cast<uint32_t>(a.nz);
+    uint32_t xr, yr, zr;
+    const uint32_t cell = linear_cell(a.nx, a.ny, a.inv_nx, a.inv_ny, xr, yr, zr);
     active = cell < n;
     if constexpr (!REPORT) {
       if (!active) return;
     }
-    uint32_t xr, yr;
-    const uint32_t drow = div_small(x0 + threadIdx.x, nx, a.inv_nx, xr);
-    const uint32_t dz = div_small(y0 + drow, ny, a.inv_ny, yr);
     // report variant: lanes past the last cell mirror it (in-bounds loads) and store nothing
     x = active ? static_cast<int>(xr) : a.nx - 1;
     y = active ? static_cast<int>(yr) : a.ny - 1;
-    z = active ? static_cast<int>(z0 + dz) : a.nz - 1;
+    z = active ? static_cast<int>(zr) : a.nz - 1;
     idx = active ? cell : n - 1;
   } else {
     uint32_t xb, row;

@@ -41,8 +41,9 @@ def _random_case(dims, seed):
     return phi, f0
 
 
+# rows of 64 cells take the row mapping, the others the linear one
 @pytest.mark.parametrize("dims,steps,window,k", [((12, 9, 8), 24, 10, 5), ((7, 5, 6), 13, 13, 4),
-                                                 ((33, 4, 3), 9, 3, 100)])
+                                                 ((33, 4, 3), 9, 3, 100), ((64, 4, 3), 9, 4, 5)])
 def test_adjoint_matches_fp64_autograd_on_every_cell(dims, steps, window, k):
     """Full-lattice block: dJ/dphi of every cell, with the sponge overlapping part of the
     solid fraction (both branches of sigma = max(phi, k t^2)) and x, y, z wrapping."""
