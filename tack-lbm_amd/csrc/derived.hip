@@ -15,6 +15,7 @@ namespace tlbm {
 // d u_c / d x_d at every point: central differences, one-sided at the grid's faces (a face
 // point differences with its one neighbour over one spacing), zero along an axis of one point
 // (derived.py:52-89).  jac holds 9 planes of m floats: jac[(3 c + d) m + p].
+namespace {
 __global__ void __launch_bounds__(256)
     velocity_gradient_kernel(const float* __restrict__ ux, const float* __restrict__ uy,
                              const float* __restrict__ uz, float* __restrict__ jac, int mx, int my,
@@ -45,6 +46,7 @@ __global__ void __launch_bounds__(256)
     jac[(3 * c + 2) * m + p] = (u[c][z1] - u[c][z0]) * sz;
   }
 }
+}  // namespace
 
 // |curl u| and Q = -1/2 (G_00^2 + G_11^2 + G_22^2) - (G_01 G_10 + G_02 G_20 + G_12 G_21) at each
 // cell centre, with G the cell's velocity gradient: BASIS = false, the mean of its eight

@@ -83,8 +83,10 @@ void fill_heat_table(HeatTable& t) {
 
 // One per (storage, collision) unit; defined in step_inst.hip compiled with
 // -DTLBM_STORAGE=<s> -DTLBM_COLL=<c>.
-#define TLBM_DECLARE_UNIT(s, c) \
-  void tlbm_register_##s##_##c(tlbm::StepTable& t, tlbm::BatchTable& bt);
+// (library-internal: hidden from the shared object's exports)
+#define TLBM_DECLARE_UNIT(s, c)                       \
+  __attribute__((visibility("hidden"))) void tlbm_register_##s##_##c(tlbm::StepTable& t, \
+                                                                      tlbm::BatchTable& bt);
 TLBM_DECLARE_UNIT(0, 0)
 TLBM_DECLARE_UNIT(0, 1)
 TLBM_DECLARE_UNIT(0, 2)
@@ -96,7 +98,8 @@ TLBM_DECLARE_UNIT(2, 1)
 TLBM_DECLARE_UNIT(2, 2)
 
 // One per collision; defined in heat_inst.hip compiled with -DTLBM_COLL=<c>.
-#define TLBM_DECLARE_HEAT_UNIT(c) void tlbm_register_heat_##c(tlbm::HeatTable& t);
+#define TLBM_DECLARE_HEAT_UNIT(c) \
+  __attribute__((visibility("hidden"))) void tlbm_register_heat_##c(tlbm::HeatTable& t);
 TLBM_DECLARE_HEAT_UNIT(0)
 TLBM_DECLARE_HEAT_UNIT(1)
 TLBM_DECLARE_HEAT_UNIT(2)
