@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) lbm_adjoint_kernel(const AdjArgs a) {
   uint32_t idx;
   if constexpr (LIN) {
     uint32_t xr, yr, zr;
-    idx = linear_cell(snx, ny, a.inv_nx, a.inv_ny, xr, yr, zr);
+    idx = linear_cell(blockIdx.x * blockDim.x, snx, ny, a.inv_nx, a.inv_ny, xr, yr, zr);
     if (idx >= plane * static_cast<uint32_t>(nz)) return;
     x = static_cast<int>(xr);
     y = static_cast<int>(yr);

@@ -392,6 +392,7 @@ int batch_build(tlbm_batch* b) {
     D.cell_off = static_cast<uint32_t>(cell);
     D.chunk_off = static_cast<uint32_t>(chunk);
     D.inv_nx = 1.0f / static_cast<float>(p.nx);
+    D.inv_ny = 1.0f / static_cast<float>(p.ny);
     D.inv_nxny = 1.0f / static_cast<float>(static_cast<uint64_t>(p.nx) * p.ny);
     b->chunk_off[d] = static_cast<uint32_t>(chunk);
     elem += 19 * static_cast<uint64_t>(p.qstride);
@@ -788,9 +789,9 @@ int tlbm_batch_create(int handle, const tlbm_params* params, int ndomains, tlbm_
     const int64_t n = static_cast<int64_t>(p.nx) * p.ny * p.nz;
     if (p.qstride < n) p.qstride = n;  // a plane stride below the cell count means "packed"
     if (int rc = check_params(&p, d.obstacle)) return rc;
-    if (n >= (1 << 24))
-      return fail(TLBM_ERR_INVALID, "domain %d has %lld cells; batched domains hold < 2^24", i,
-                  static_cast<long long>(n));
+    if (p.nx > (1 << 24) - 256 || p.ny > (1 << 24) - 256)
+      return fail(TLBM_ERR_INVALID, "domain %d: nx and ny must be at most 2^24 - 256 (got %d, %d)",
+                  i, p.nx, p.ny);
   }
   tlbm_batch* b = new tlbm_batch();
   b->handle = handle - d.diag;

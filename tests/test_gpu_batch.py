@@ -175,22 +175,23 @@ def test_sweep_domains_are_seeded_and_in_range():
         assert 100 <= d.reynolds <= 1000 and 8 <= d.diameter <= 24
 
 
-def test_batch_domain_near_the_cell_limit_steps_as_the_single_lattice():
-    """A batch of one 256 x 256 x 255 domain (16,711,680 cells, just under the 2^24 a batched
-    domain may hold: the largest cell indices for the float-reciprocal index split) plus a
+def test_large_batch_domain_steps_as_the_single_lattice():
+    """A batch of one 256 x 256 x 300 domain (19,660,800 cells: beyond the 2^24 cells a float
+    reciprocal splits exactly, so the index split must work from the chunk's first row) plus a
     small one steps as the single-lattice kernel (LES, FP32, from the same seeded initial
     state): within 2 ulp, the bar of the batch parity test (the two kernels contract the
     sponge blend of q = 0 into FMAs differently), and bit for bit outside the sponge."""
     import tlbm
     from tlbm.batch import BatchSolver, Domain
-    big = Domain(256, 256, 255, 3000.0, 32.0, center=(96.0, 128.0, 127.5))
+    big = Domain(256, 256, 300, 3000.0, 32.0, center=(96.0, 128.0, 150.0))
+    assert big.n > 2 ** 24
     b = BatchSolver([Domain(20, 10, 10, 150.0, 5.0), big], collision="les", obstacle="sphere",
                     lateral="lid", storage="f32", noise_amp=1e-3)
     b.step()
     tlbm.sync()
     got = _view(b.f[0].numpy(), b, 1)
-    s = tlbm.Solver(256, 256, 255, collision="les", obstacle="sphere", reynolds=3000.0,
-                    diameter=32.0, center=(96.0, 128.0, 127.5), lateral="lid", noise_amp=1e-3,
+    s = tlbm.Solver(256, 256, 300, collision="les", obstacle="sphere", reynolds=3000.0,
+                    diameter=32.0, center=(96.0, 128.0, 150.0), lateral="lid", noise_amp=1e-3,
                     qstride=b.plane_strides[1])
     # the Solver takes one report step at construction (lbm.py:583-584); restart it from the
     # batch's initial state
