@@ -46,6 +46,7 @@ struct AdjArgs {
   double* gphi;          // dJ/dphi over the parameter block, x fastest
   uint32_t xblocks, rows;  // row mapping
   float inv_nx, inv_ny;    // linear mapping (LIN), as in lbm_step_kernel
+  Divisor div_nx, div_ny;
 };
 
 template <bool FINAL, bool LIN>
@@ -56,7 +57,8 @@ __global__ void __launch_bounds__(256) lbm_adjoint_kernel(const AdjArgs a) {
   uint32_t idx;
   if constexpr (LIN) {
     uint32_t xr, yr, zr;
-    idx = linear_cell(blockIdx.x * blockDim.x, snx, ny, a.inv_nx, a.inv_ny, xr, yr, zr);
+    idx = linear_cell(blockIdx.x * blockDim.x, snx, ny, a.div_nx, a.div_ny, a.inv_nx, a.inv_ny,
+                      xr, yr, zr);
     if (idx >= plane * static_cast<uint32_t>(nz)) return;
     x = static_cast<int>(xr);
     y = static_cast<int>(yr);

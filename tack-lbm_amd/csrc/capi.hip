@@ -228,6 +228,10 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
   a.nx = p.nx;
   a.ny = p.ny;
   a.nz = p.nz;
+  a.inv_nx = 1.0f / static_cast<float>(p.nx);
+  a.inv_ny = 1.0f / static_cast<float>(p.ny);
+  a.div_nx = tlbm::make_divisor(static_cast<uint32_t>(p.nx));
+  a.div_ny = tlbm::make_divisor(static_cast<uint32_t>(p.ny));
   a.omega = p.omega;
   a.tau0 = p.tau0;
   a.smag = 25.456f * p.cs * p.cs;
@@ -671,8 +675,6 @@ int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const
     bx = g_block_x[handle] ? g_block_x[handle] : 256;
     nblocks64 = (rows * params->nx + bx - 1) / bx;
     a.linear = 1;
-    a.inv_nx = 1.0f / static_cast<float>(params->nx);
-    a.inv_ny = 1.0f / static_cast<float>(params->ny);
   } else {
     bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(params->nx, d.storage);
     const uint64_t xblocks = (params->nx + bx - 1) / bx;
@@ -1032,6 +1034,8 @@ int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* 
   a.rows = static_cast<uint32_t>(rows);
   a.inv_nx = 1.0f / static_cast<float>(params->nx);
   a.inv_ny = 1.0f / static_cast<float>(params->ny);
+  a.div_nx = tlbm::make_divisor(static_cast<uint32_t>(params->nx));
+  a.div_ny = tlbm::make_divisor(static_cast<uint32_t>(params->ny));
   if (nblocks * bx > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
   const dim3 grid(static_cast<uint32_t>(nblocks)), block(bx);
   hipStream_t st = static_cast<hipStream_t>(stream);

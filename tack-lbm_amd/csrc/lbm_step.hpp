@@ -112,6 +112,27 @@ struct StoreBF16 {
   }
 };
 
+// ---- unsigned division by a divisor fixed for the launch ----------------------------------
+// Granlund and Montgomery, "Division by invariant integers using multiplication" (PLDI 1994),
+// section 4: with l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, sh1 = min(l, 1) and
+// sh2 = max(l - 1, 0), n / d = (t + ((n - t) >> sh1)) >> sh2 where t = mulhi(m, n), for every
+// 32-bit n and 1 <= d < 2^31.  A block-uniform n compiles to a few scalar instructions.
+struct Divisor {
+  uint32_t m, sh1, sh2;
+};
+
+inline Divisor make_divisor(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  const uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+  return Divisor{static_cast<uint32_t>(m), l < 1 ? l : 1u, l > 1 ? l - 1 : 0u};
+}
+
+__device__ __forceinline__ uint32_t divide(uint32_t n, const Divisor& d) {
+  const uint32_t t = static_cast<uint32_t>((static_cast<uint64_t>(d.m) * n) >> 32);
+  return (t + ((n - t) >> d.sh1)) >> d.sh2;
+}
+
 // ---- kernel arguments (host fills them from tlbm_params) ----------------------------------
 struct StepArgs {
   const void* fin;
@@ -135,6 +156,7 @@ struct StepArgs {
   uint32_t xblocks, rows;    // launch shape: blocks per x-row, ny * nz rows
   uint32_t linear;           // 1: launch the linear-mapping kernel (host side)
   float inv_nx, inv_ny;      // 1/nx, 1/ny for the linear mapping's index split
+  Divisor div_nx, div_ny;    // the same divisors for block-uniform numerators
   // temperature variant (lbm.py:457-492)
   const float* g_in;
   float* g_out;
@@ -779,11 +801,12 @@ __device__ __forceinline__ uint32_t div_small(uint32_t i, uint32_t d, float inv,
 // count.  div_small's range needs nx + 256 and ny + 256 below 2^24 (checked on the host); the
 // index itself may take any 32-bit value.
 __device__ __forceinline__ uint32_t linear_cell(uint32_t c0, uint32_t nx, uint32_t ny,
+                                                const Divisor& div_nx, const Divisor& div_ny,
                                                 float inv_nx, float inv_ny, uint32_t& x,
                                                 uint32_t& y, uint32_t& z) {
-  const uint32_t row0 = c0 / nx;
+  const uint32_t row0 = divide(c0, div_nx);
   const uint32_t x0 = c0 - row0 * nx;
-  const uint32_t z0 = row0 / ny;
+  const uint32_t z0 = divide(row0, div_ny);
   const uint32_t y0 = row0 - z0 * ny;
   const uint32_t drow = div_small(x0 + threadIdx.x, nx, inv_nx, x);
   z = z0 + div_small(y0 + drow, ny, inv_ny, y);
@@ -810,7 +833,8 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
     const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * static_cast<uint32_t>(a.nz);
     uint32_t xr, yr, zr;
     const uint32_t cell =
-        linear_cell(blockIdx.x * blockDim.x, a.nx, a.ny, a.inv_nx, a.inv_ny, xr, yr, zr);
+        linear_cell(blockIdx.x * blockDim.x, a.nx, a.ny, a.div_nx, a.div_ny, a.inv_nx,
+                    a.inv_ny, xr, yr, zr);
     active = cell < n;
     if constexpr (!REPORT) {
       if (!active) return;
@@ -898,7 +922,7 @@ __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
       y = div_small(yz, static_cast<uint32_t>(a.nx), D.inv_nx, x);
     } else {
       uint32_t xr, yr, zr;
-      local = linear_cell(c0, a.nx, a.ny, D.inv_nx, D.inv_ny, xr, yr, zr);
+      local = linear_cell(c0, a.nx, a.ny, a.div_nx, a.div_ny, D.inv_nx, D.inv_ny, xr, yr, zr);
       active = local < n;
       cell = active ? local : n - 1;
       x = active ? xr : a.nx - 1;
