@@ -912,8 +912,8 @@ __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
     uint32_t local, cell, x, y, z;
     bool active;
     if (n < (1u << 24)) {
-      // below 2^24 cells the whole index splits with float reciprocals: on MI355X 2.7% faster
-      // for the 16-bit sweep than linear_cell's block-uniform division at every chunk start
+      // below 2^24 cells the whole index splits with float reciprocals (on MI355X 2.7% faster
+      // for the 16-bit sweep than a compiler 32-bit division at every chunk start)
       local = c0 + threadIdx.x;
       active = local < n;
       cell = active ? local : n - 1;
