@@ -45,8 +45,7 @@ struct AdjArgs {
   int32_t bn[3];         // parameter block size
   double* gphi;          // dJ/dphi over the parameter block, x fastest
   uint32_t xblocks, rows;  // row mapping
-  float inv_nx, inv_ny;    // linear mapping (LIN), as in lbm_step_kernel
-  Divisor div_nx, div_ny;
+  Divisor div_nx, div_ny;  // linear mapping (LIN): cell index -> (x, y, z)
 };
 
 template <bool FINAL, bool LIN>
@@ -57,9 +56,9 @@ __global__ void __launch_bounds__(256) lbm_adjoint_kernel(const AdjArgs a) {
   uint32_t idx;
   if constexpr (LIN) {
     uint32_t xr, yr, zr;
-    idx = linear_cell(blockIdx.x * blockDim.x, snx, ny, a.div_nx, a.div_ny, a.inv_nx, a.inv_ny,
-                      xr, yr, zr);
+    idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= plane * static_cast<uint32_t>(nz)) return;
+    cell_coords(idx, snx, ny, a.div_nx, a.div_ny, xr, yr, zr);
     x = static_cast<int>(xr);
     y = static_cast<int>(yr);
     z = static_cast<int>(zr);

@@ -228,8 +228,6 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
   a.nx = p.nx;
   a.ny = p.ny;
   a.nz = p.nz;
-  a.inv_nx = 1.0f / static_cast<float>(p.nx);
-  a.inv_ny = 1.0f / static_cast<float>(p.ny);
   a.div_nx = tlbm::make_divisor(static_cast<uint32_t>(p.nx));
   a.div_ny = tlbm::make_divisor(static_cast<uint32_t>(p.ny));
   a.omega = p.omega;
@@ -276,20 +274,15 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
   return a;
 }
 
-// Threads per block along x: of 256, 192, 128 and 64, the one that launches the fewest lanes
-// for a row (ceil(nx / b) * b), the larger on a tie.  Measured on MI355X over rows of 96 to 1024
-// cells (LES, FP32 and FP16), this picks the fastest block or one within 2% of it; 256 on rows
-// it does not divide costs up to 25% (384 cells: a second block half idle).
-// The step kernel's cell mapping: the row mapping (blocks along x-rows, y and z uniform per
-// block) where rows are a multiple of 64 cells, else the linear one (blocks of consecutive
-// cells), so that every wave stores whole aligned cache lines.  Measured on MI355X at 33.5 M
-// cells, rows of 500 cells took 1.347 ms (FP32) and 0.707 ms (FP16) per step with the row
-// mapping against 0.897 and 0.462 ms for rows of 512.  The linear mapping's index split
-// (div_small) needs nx + 256 and ny + 256 below 2^24.
-bool linear_mapping(int nx, int ny) {
-  return nx % 64 != 0 && nx <= (1 << 24) - 256 && ny <= (1 << 24) - 256;
-}
+// The adjoint kernel's cell mapping: blocks along x-rows (y and z uniform per block) where
+// rows are a multiple of 64 cells, else blocks of consecutive cells, so that every wave stores
+// whole aligned cache lines.  The step kernel maps consecutive cells always (lbm_step_kernel).
+bool linear_mapping(int nx) { return nx % 64 != 0; }
 
+// Threads per block along x for a row mapping: of 256, 192, 128 and 64, the one that launches
+// the fewest lanes for a row (ceil(nx / b) * b), the larger on a tie.  Measured on MI355X for
+// the step over rows of 96 to 1024 cells (LES, FP32 and FP16, when it mapped rows), this picked
+// the fastest block or one within 2% of it; 256 on rows it does not divide cost up to 25%.
 int default_block_x(int nx, int) {
   int best = 256;
   int64_t lanes = (static_cast<int64_t>(nx) + 255) / 256 * 256;
@@ -395,9 +388,6 @@ int batch_build(tlbm_batch* b) {
     D.elem_off = elem;
     D.cell_off = static_cast<uint32_t>(cell);
     D.chunk_off = static_cast<uint32_t>(chunk);
-    D.inv_nx = 1.0f / static_cast<float>(p.nx);
-    D.inv_ny = 1.0f / static_cast<float>(p.ny);
-    D.inv_nxny = 1.0f / static_cast<float>(static_cast<uint64_t>(p.nx) * p.ny);
     b->chunk_off[d] = static_cast<uint32_t>(chunk);
     elem += 19 * static_cast<uint64_t>(p.qstride);
     cell += n;
@@ -668,23 +658,13 @@ int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const
     a.nu_mol = heat->nu;
     a.pr_t = heat->pr_t;
   }
-  const uint64_t rows = static_cast<uint64_t>(params->ny) * params->nz;
-  uint64_t nblocks64;
-  int bx;
-  if (linear_mapping(params->nx, params->ny)) {
-    bx = g_block_x[handle] ? g_block_x[handle] : 256;
-    nblocks64 = (rows * params->nx + bx - 1) / bx;
-    a.linear = 1;
-  } else {
-    bx = g_block_x[handle] ? g_block_x[handle] : default_block_x(params->nx, d.storage);
-    const uint64_t xblocks = (params->nx + bx - 1) / bx;
-    nblocks64 = xblocks * rows;
-    a.xblocks = static_cast<uint32_t>(xblocks);
-  }
+  // one thread per cell, blocks of consecutive cells in index order (lbm_step_kernel)
+  const int bx = g_block_x[handle] ? g_block_x[handle] : 256;
+  const uint64_t n = static_cast<uint64_t>(params->nx) * params->ny * params->nz;
+  const uint64_t nblocks64 = (n + bx - 1) / bx;
   if (nblocks64 * bx > 0xFFFFFFFFull)
     return fail(TLBM_ERR_INVALID, "grid of %llu threads exceeds the launch limit",
                 static_cast<unsigned long long>(nblocks64 * bx));
-  a.rows = static_cast<uint32_t>(rows);
   const dim3 block(bx);
   const dim3 grid(static_cast<uint32_t>(nblocks64));
   const uint32_t nblocks = grid.x * grid.y * grid.z;
@@ -791,9 +771,6 @@ int tlbm_batch_create(int handle, const tlbm_params* params, int ndomains, tlbm_
     const int64_t n = static_cast<int64_t>(p.nx) * p.ny * p.nz;
     if (p.qstride < n) p.qstride = n;  // a plane stride below the cell count means "packed"
     if (int rc = check_params(&p, d.obstacle)) return rc;
-    if (p.nx > (1 << 24) - 256 || p.ny > (1 << 24) - 256)
-      return fail(TLBM_ERR_INVALID, "domain %d: nx and ny must be at most 2^24 - 256 (got %d, %d)",
-                  i, p.nx, p.ny);
   }
   tlbm_batch* b = new tlbm_batch();
   b->handle = handle - d.diag;
@@ -1025,15 +1002,13 @@ int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* 
   // the cell mapping of the forward step: rows that are a multiple of 64 cells take the row
   // mapping, with blocks of 64/128 threads on rows of up to 128 cells (a 256-thread block on a
   // 128-cell row would leave half its waves idle); other rows the linear mapping
-  const bool linear = linear_mapping(params->nx, params->ny);
+  const bool linear = linear_mapping(params->nx);
   const uint64_t rows = static_cast<uint64_t>(params->ny) * params->nz;
   const int bx = linear ? 256 : default_block_x(params->nx, TLBM_F32);
   const uint64_t xblocks = (params->nx + bx - 1) / bx;
   const uint64_t nblocks = linear ? (rows * params->nx + bx - 1) / bx : xblocks * rows;
   a.xblocks = static_cast<uint32_t>(xblocks);
   a.rows = static_cast<uint32_t>(rows);
-  a.inv_nx = 1.0f / static_cast<float>(params->nx);
-  a.inv_ny = 1.0f / static_cast<float>(params->ny);
   a.div_nx = tlbm::make_divisor(static_cast<uint32_t>(params->nx));
   a.div_ny = tlbm::make_divisor(static_cast<uint32_t>(params->ny));
   if (nblocks * bx > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");

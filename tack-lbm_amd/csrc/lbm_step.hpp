@@ -153,10 +153,8 @@ struct StepArgs {
   tlbm_fields fld;
   double* partials;          // report: per-block FP64 partial sums [TLBM_NSUMS][nblocks]
   uint32_t nblocks;
-  uint32_t xblocks, rows;    // launch shape: blocks per x-row, ny * nz rows
-  uint32_t linear;           // 1: launch the linear-mapping kernel (host side)
-  float inv_nx, inv_ny;      // 1/nx, 1/ny for the linear mapping's index split
-  Divisor div_nx, div_ny;    // the same divisors for block-uniform numerators
+  uint32_t xblocks, rows;    // row-mapped launches (lbm_init_kernel): blocks per x-row, rows
+  Divisor div_nx, div_ny;    // cell index -> (x, y, z)
   // temperature variant (lbm.py:457-492)
   const float* g_in;
   float* g_out;
@@ -778,91 +776,44 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   }
 }
 
-// q = i / d, r = i % d for i < 2^24, with a float reciprocal and one correction each way.
-__device__ __forceinline__ uint32_t div_small(uint32_t i, uint32_t d, float inv, uint32_t& r) {
-  uint32_t q = static_cast<uint32_t>(static_cast<float>(i) * inv);
-  int32_t rr = static_cast<int32_t>(i) - static_cast<int32_t>(q * d);
-  if (rr < 0) {
-    q -= 1;
-    rr += static_cast<int32_t>(d);
-  }
-  if (rr >= static_cast<int32_t>(d)) {
-    q += 1;
-    rr -= static_cast<int32_t>(d);
-  }
-  r = static_cast<uint32_t>(rr);
-  return q;
-}
-
-// The linear mapping's cell of this lane: a block takes cells [c0, c0 + B) in index order (c0
-// block-uniform).  The block's first row (row0 = z0 * ny + y0, column x0) is block-uniform;
-// lane t lies (x0 + t) / nx rows on.  Returns the cell index with its (x, y, z); lanes past the
-// last cell get coordinates beyond the lattice, so the caller tests the index against the cell
-// count.  div_small's range needs nx + 256 and ny + 256 below 2^24 (checked on the host); the
-// index itself may take any 32-bit value.
-__device__ __forceinline__ uint32_t linear_cell(uint32_t c0, uint32_t nx, uint32_t ny,
-                                                const Divisor& div_nx, const Divisor& div_ny,
-                                                float inv_nx, float inv_ny, uint32_t& x,
-                                                uint32_t& y, uint32_t& z) {
-  const uint32_t row0 = divide(c0, div_nx);
-  const uint32_t x0 = c0 - row0 * nx;
-  const uint32_t z0 = divide(row0, div_ny);
-  const uint32_t y0 = row0 - z0 * ny;
-  const uint32_t drow = div_small(x0 + threadIdx.x, nx, inv_nx, x);
-  z = z0 + div_small(y0 + drow, ny, inv_ny, y);
-  return c0 + threadIdx.x;
+// (x, y, z) of a cell index (x fastest): two divisions by invariant integers, exact for every
+// 32-bit index.
+__device__ __forceinline__ void cell_coords(uint32_t cell, uint32_t nx, uint32_t ny,
+                                            const Divisor& div_nx, const Divisor& div_ny,
+                                            uint32_t& x, uint32_t& y, uint32_t& z) {
+  const uint32_t row = divide(cell, div_nx);
+  x = cell - row * nx;
+  z = divide(row, div_ny);
+  y = row - z * ny;
 }
 
 // ---- the step kernel of one lattice -------------------------------------------------------------
 // HEAT: with the temperature step (FP32 populations only).
-// LIN: the linear mapping.  Block b takes cells [b * B, b * B + B) in index order, so every
-// wave reads and writes whole aligned cache lines of each plane whatever nx is; y and z are
-// per lane.  The row mapping (one block per part of an x-row) starts each row's waves at the
-// row's first cell: when nx is not a multiple of 64 those runs straddle cache lines that the
-// neighbouring rows, on other XCDs, write too, which measured 1.5x slower on MI355X.  The host
-// takes LIN for such rows.
+// Block b takes cells [b * B, b * B + B) in index order, so every wave reads and writes whole
+// aligned cache lines of each plane whatever nx is; y and z are per lane.  Measured on MI355X
+// against blocks along x-rows (y and z uniform per block, row blocks kept on one XCD): equal or
+// up to 4% faster on rows that are a multiple of 64 cells, and 1.5-2x faster on rows that are
+// not (their waves started off a cache line, and two XCDs wrote the line a row ends in).
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool HEAT = false,
-          bool NT = false, bool LIN = false>
+          bool NT = false>
 __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   static_assert(!HEAT || STORAGE == TLBM_F32, "the temperature variant stores FP32");
   using T = typename StorageT<STORAGE>::T;
-  int x, y, z;
-  uint32_t idx;
-  bool active;
-  if constexpr (LIN) {
-    const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * static_cast<uint32_t>(a.nz);
-    uint32_t xr, yr, zr;
-    const uint32_t cell =
-        linear_cell(blockIdx.x * blockDim.x, a.nx, a.ny, a.div_nx, a.div_ny, a.inv_nx,
-                    a.inv_ny, xr, yr, zr);
-    active = cell < n;
-    if constexpr (!REPORT) {
-      if (!active) return;
-    }
-    // report variant: lanes past the last cell mirror it (in-bounds loads) and store nothing
-    x = active ? static_cast<int>(xr) : a.nx - 1;
-    y = active ? static_cast<int>(yr) : a.ny - 1;
-    z = active ? static_cast<int>(zr) : a.nz - 1;
-    idx = active ? cell : n - 1;
-  } else {
-    uint32_t xb, row;
-    tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
-    const int xr = static_cast<int>(xb * blockDim.x + threadIdx.x);
-    active = xr < a.nx;
-    if constexpr (!REPORT) {
-      if (!active) return;
-    }
-    // in the report variant every lane reaches the block reduction: lanes past the end of the
-    // row mirror the last cell (in-bounds loads) and store nothing
-    x = active ? xr : a.nx - 1;
-    y = static_cast<int>(row % static_cast<uint32_t>(a.ny));
-    z = static_cast<int>(row / static_cast<uint32_t>(a.ny));
-    idx = (static_cast<uint32_t>(z) * a.ny + y) * static_cast<uint32_t>(a.nx) + x;
+  const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * static_cast<uint32_t>(a.nz);
+  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = cell < n;
+  if constexpr (!REPORT) {
+    if (!active) return;
   }
+  // in the report variant every lane reaches the block reduction: lanes past the last cell
+  // mirror it (in-bounds loads) and store nothing
+  const uint32_t idx = active ? cell : n - 1;
+  uint32_t x, y, z;
+  cell_coords(idx, a.nx, a.ny, a.div_nx, a.div_ny, x, y, z);
   double acc[TLBM_NSUMS];
-  step_cell<COLL, OBS, SIDE, REPORT, STORAGE, HEAT, NT>(a, static_cast<const T*>(a.fin),
-                                                    static_cast<T*>(a.fout), x, y, z, active,
-                                                    a.fld, idx, acc);
+  step_cell<COLL, OBS, SIDE, REPORT, STORAGE, HEAT, NT>(
+      a, static_cast<const T*>(a.fin), static_cast<T*>(a.fout), static_cast<int>(x),
+      static_cast<int>(y), static_cast<int>(z), active, a.fld, idx, acc);
   if constexpr (REPORT) {
     if (a.partials) block_sums(acc, a.partials, a.nblocks, blockIdx.x);
   }
@@ -878,8 +829,6 @@ struct BatchDomain {
   uint64_t elem_off;   // first population element of the domain
   uint32_t cell_off;   // first cell of the domain in packed per-cell arrays
   uint32_t chunk_off;  // first chunk of the domain
-  float inv_nx, inv_ny, inv_nxny;  // for the idx -> (x, y, z) split
-  uint32_t pad;
 };
 
 struct BatchArgs {
@@ -908,27 +857,11 @@ __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
     const StepArgs& a = D.a;
     const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * a.nz;
     // lanes past the domain's last cell mirror it (in-bounds loads) and store nothing
-    const uint32_t c0 = (c - D.chunk_off) * blockDim.x;
-    uint32_t local, cell, x, y, z;
-    bool active;
-    if (n < (1u << 24)) {
-      // below 2^24 cells the whole index splits with float reciprocals (on MI355X 2.7% faster
-      // for the 16-bit sweep than a compiler 32-bit division at every chunk start)
-      local = c0 + threadIdx.x;
-      active = local < n;
-      cell = active ? local : n - 1;
-      uint32_t yz;
-      z = div_small(cell, static_cast<uint32_t>(a.nx) * a.ny, D.inv_nxny, yz);
-      y = div_small(yz, static_cast<uint32_t>(a.nx), D.inv_nx, x);
-    } else {
-      uint32_t xr, yr, zr;
-      local = linear_cell(c0, a.nx, a.ny, a.div_nx, a.div_ny, D.inv_nx, D.inv_ny, xr, yr, zr);
-      active = local < n;
-      cell = active ? local : n - 1;
-      x = active ? xr : a.nx - 1;
-      y = active ? yr : a.ny - 1;
-      z = active ? zr : a.nz - 1;
-    }
+    const uint32_t local = (c - D.chunk_off) * blockDim.x + threadIdx.x;
+    const bool active = local < n;
+    const uint32_t cell = active ? local : n - 1;
+    uint32_t x, y, z;
+    cell_coords(cell, a.nx, a.ny, a.div_nx, a.div_ny, x, y, z);
     double acc[TLBM_NSUMS];
     step_cell<COLL, OBS, SIDE, REPORT, STORAGE, false, NT>(a, fin + D.elem_off, fout + D.elem_off,
                                                 static_cast<int>(x), static_cast<int>(y),

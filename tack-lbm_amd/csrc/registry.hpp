@@ -17,25 +17,17 @@ using HeatTable = StepLaunch[kObstacles][3][2];
 
 // Quiet steps come in two builds: plain and non-temporal population stores (NT), chosen per
 // launch by a.stream_stores (the host sets it for large lattices).  Report steps always use
-// plain stores.  Each comes with the row and the linear cell mapping (a.linear, set by the
-// host for rows that are not a multiple of 64 cells).
-template <int C, int O, int S, bool R, int ST, bool H, bool LIN>
-hipError_t launch_mapped(const StepArgs& a, dim3 grid, dim3 block, hipStream_t stream) {
+// plain stores.
+template <int C, int O, int S, bool R, int ST, bool H = false>
+hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t stream) {
   if constexpr (!R) {
     if (a.stream_stores) {
-      hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, true, LIN>), grid, block, 0, stream,
-                         a);
+      hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, true>), grid, block, 0, stream, a);
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, false, LIN>), grid, block, 0, stream, a);
+  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, false>), grid, block, 0, stream, a);
   return hipGetLastError();
-}
-
-template <int C, int O, int S, bool R, int ST, bool H = false>
-hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t stream) {
-  return a.linear ? launch_mapped<C, O, S, R, ST, H, true>(a, grid, block, stream)
-                  : launch_mapped<C, O, S, R, ST, H, false>(a, grid, block, stream);
 }
 
 using BatchLaunch = hipError_t (*)(const BatchArgs&, bool nt, dim3 grid, dim3 block,

@@ -176,11 +176,11 @@ def test_sweep_domains_are_seeded_and_in_range():
 
 
 def test_large_batch_domain_steps_as_the_single_lattice():
-    """A batch of one 256 x 256 x 300 domain (19,660,800 cells: beyond the 2^24 cells a float
-    reciprocal splits exactly, so the index split must work from the chunk's first row) plus a
-    small one steps as the single-lattice kernel (LES, FP32, from the same seeded initial
-    state): within 2 ulp, the bar of the batch parity test (the two kernels contract the
-    sponge blend of q = 0 into FMAs differently), and bit for bit outside the sponge."""
+    """A batch of one 256 x 256 x 300 domain (19,660,800 cells, beyond 2^24: the index split
+    divides by invariant integers, exact for any 32-bit index) plus a small one steps as the
+    single-lattice kernel (LES, FP32, from the same seeded initial state): within 2 ulp, the
+    bar of the batch parity test (the two kernels may contract the sponge blend of q = 0 into
+    FMAs differently), and bit for bit outside the sponge."""
     import tlbm
     from tlbm.batch import BatchSolver, Domain
     big = Domain(256, 256, 300, 3000.0, 32.0, center=(96.0, 128.0, 150.0))

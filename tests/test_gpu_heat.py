@@ -68,7 +68,7 @@ COMBOS = [(c, o, s) for c in ("bgk", "trt", "les")
           for s in ("periodic", "slip", "lid")]
 
 
-# rows of 24 cells take the linear cell mapping, rows of 64 the row mapping
+# rows of 24 cells (waves straddle rows) and of 64 (waves within a row)
 @pytest.mark.parametrize("nx", [24, 64])
 @pytest.mark.parametrize("report", [False, True])
 @pytest.mark.parametrize("collision,obstacle,side", COMBOS)
@@ -173,8 +173,8 @@ def test_large_heat_lattice_repeats_the_small_one(nx):
     """A temperature step on a lattice large enough for the non-temporal-store kernel (nx x 64 x
     1024, 26 FP32 planes of about 4 M cells in each buffer) whose state, solid temperatures and
     geometry (a z-axis cylinder) repeat every 4 cells along z gives, bit for bit, the step of
-    the nx x 64 x 4 lattice (plain stores) repeated 256 times.  nx = 64: the row mapping;
-    nx = 60: the linear one."""
+    the nx x 64 x 4 lattice (plain stores) repeated 256 times.  nx = 64: waves within rows;
+    nx = 60: waves and blocks straddling rows."""
     import tlbm
     reps = 256
     small = Case((nx, 64, 4), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))

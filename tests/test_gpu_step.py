@@ -83,7 +83,7 @@ COMBOS = [(c, o, s) for c in ("bgk", "trt", "les")
           for s in ("periodic", "slip", "lid")]
 
 
-# Rows of 24 cells take the linear cell mapping, rows of 64 the row mapping.
+# Rows of 24 cells (waves straddle rows) and of 64 (waves within a row).
 MAPPING_NX = [24, 64]
 
 
@@ -163,8 +163,8 @@ def test_planes_beyond_32_bit_offsets(storage, nx):
     rotating z-axis cylinder) repeat every 4 cells along z, so a report step and then a quiet
     step must give, bit for bit, the steps of the nx x 64 x 4 lattice repeated 8192 times, and
     FP64 sums 8192 times that lattice's.  The large lattice's quiet step takes the
-    non-temporal-store kernel, the small one's the plain one; nx = 64 runs the row mapping,
-    nx = 60 the linear one (blocks of 256 cells straddle rows and planes)."""
+    non-temporal-store kernel, the small one's the plain one; with nx = 60 blocks of 256 cells
+    straddle rows and planes."""
     import tlbm
     reps = 8192
     small = Case((nx, 64, 4), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
