@@ -123,10 +123,8 @@ int tlbm_event_elapsed_ms(void* start, void* stop, float* ms);
 /* Resolve a specialisation (Tack's type-keyed kernel cache, README.md:55-56).  The handle is
  * stable for the life of the process. */
 int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle);
-/* Launch shape of a specialisation: threads per block (64, 128, 192 or 256).  Where nx is a
- * multiple of 64 a block covers part of one x-row, and 0 restores the default: of those four,
- * the one launching the fewest lanes per row, the larger on a tie.  Other lattices step blocks
- * of consecutive cells in index order (every wave on aligned cache lines), default 256. */
+/* Launch shape of a specialisation: threads per block (64, 128, 192 or 256; 0 restores the
+ * default, 256).  Each block steps that many consecutive cells in index order. */
 int tlbm_set_launch(int handle, int block_x);
 
 /* lbm_init (lbm.py:383-394): f = feq(1, u0 fluid, seed_amp u0 fluid sin(2 pi x/nx)

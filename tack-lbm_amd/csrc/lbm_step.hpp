@@ -1,14 +1,11 @@
 // Device code of the fused D3Q19 step for gfx950.
 //
-// Mapping.  One thread per cell.  Where rows are a multiple of 64 cells each block covers part
-// of one x-row (tile_of_block maps block ids to rows XCD by XCD), so every y/z decision
-// (periodic wrap, slip mirror, moving lid) is uniform across the wave and every row/plane
-// offset is a scalar; other lattices take the linear mapping (blocks of consecutive cells, see
-// lbm_step_kernel), whose waves stay on aligned cache lines.  Each population plane is read and written in coalesced
-// wave-wide runs; the +-1 x offsets of the pull scheme are plain misaligned dword loads, served
-// by the same cache lines.  Measured on MI355X, this form streams as fast as 8/16-byte per-lane
-// vector loads with lane shuffles or two cells per thread, and runs the 16-bit storage formats
-// faster (fewer instructions, more waves per SIMD).  What matters most is that the 19 pull
+// Mapping.  One thread per cell; each block steps consecutive cells in index order (see
+// lbm_step_kernel), so each population plane is read and written in coalesced wave-wide runs
+// of whole aligned cache lines; the +-1 x offsets of the pull scheme are plain misaligned dword
+// loads, served by the same cache lines.  Measured on MI355X, one cell per thread streams as
+// fast as 8/16-byte per-lane vector loads with lane shuffles or two cells per thread, and runs
+// the 16-bit storage formats faster (fewer instructions, more waves per SIMD).  What matters most is that the 19 pull
 // loads of a thread are issued back to back, with no branch between them.
 //
 // Specialisation.  The kernel is a template over collision x obstacle x side x report x storage:
@@ -481,6 +478,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
 }
 
 // ---- XCD-aware block -> tile mapping -----------------------------------------------------------
+// Row-mapped kernels (lbm_init_kernel; the adjoint on rows that are a multiple of 64 cells).
 // The grid is one-dimensional: xblocks blocks per x-row, rows = ny * nz rows.  Workgroups are
 // dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8), and each XCD has its own
 // L2.  The x-blocks of one row share a cache line at every block edge (the x +- 1 pulls), so
@@ -506,9 +504,7 @@ __device__ __forceinline__ void tile_of_block(uint32_t b, uint32_t xblocks, uint
 
 // ---- the fused step of one cell ------------------------------------------------------------------
 // Pull, bounce-back, collision, sponge and store for cell (x, y, z) of the lattice described by
-// `a`, reading `fin` and writing `fout` (plane stride a.ns).  In the single-lattice kernel y and z
-// are block-uniform (scalar registers, uniform branches); in the batched kernel they are per
-// lane.  REPORT: writes the 13 fields at `fidx` (when F.ux is not null) and returns the 14 FP64
+// `a`, reading `fin` and writing `fout` (plane stride a.ns); x, y and z are per lane.  REPORT: writes the 13 fields at `fidx` (when F.ux is not null) and returns the 14 FP64
 // summands in acc (zero for inactive lanes).
 // ---- the temperature step of one cell (lbm.py:457-492) --------------------------------------------
 // D3Q7 pull, periodic in x, y and z; a link from a solid voxel bounces back, and from a sink voxel
