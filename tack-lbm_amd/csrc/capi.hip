@@ -274,28 +274,6 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
   return a;
 }
 
-// The adjoint kernel's cell mapping: blocks along x-rows (y and z uniform per block) where
-// rows are a multiple of 64 cells, else blocks of consecutive cells, so that every wave stores
-// whole aligned cache lines.  The step kernel maps consecutive cells always (lbm_step_kernel).
-bool linear_mapping(int nx) { return nx % 64 != 0; }
-
-// Threads per block along x for a row mapping: of 256, 192, 128 and 64, the one that launches
-// the fewest lanes for a row (ceil(nx / b) * b), the larger on a tie.  Measured on MI355X for
-// the step over rows of 96 to 1024 cells (LES, FP32 and FP16, when it mapped rows), this picked
-// the fastest block or one within 2% of it; 256 on rows it does not divide cost up to 25%.
-int default_block_x(int nx, int) {
-  int best = 256;
-  int64_t lanes = (static_cast<int64_t>(nx) + 255) / 256 * 256;
-  for (int b : {192, 128, 64}) {
-    const int64_t l = (static_cast<int64_t>(nx) + b - 1) / b * b;
-    if (l < lanes) {
-      lanes = l;
-      best = b;
-    }
-  }
-  return best;
-}
-
 template <int O, int S>
 hipError_t launch_init(const tlbm::StepArgs& a, float seed_amp, float noise_amp, uint64_t seed,
                        dim3 g, dim3 b, hipStream_t st) {
@@ -999,29 +977,18 @@ int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* 
                   block_origin[i], block_origin[i] + block_size[i], dims[i], i);
   }
   a.gphi = gphi;
-  // the cell mapping of the forward step: rows that are a multiple of 64 cells take the row
-  // mapping, with blocks of 64/128 threads on rows of up to 128 cells (a 256-thread block on a
-  // 128-cell row would leave half its waves idle); other rows the linear mapping
-  const bool linear = linear_mapping(params->nx);
-  const uint64_t rows = static_cast<uint64_t>(params->ny) * params->nz;
-  const int bx = linear ? 256 : default_block_x(params->nx, TLBM_F32);
-  const uint64_t xblocks = (params->nx + bx - 1) / bx;
-  const uint64_t nblocks = linear ? (rows * params->nx + bx - 1) / bx : xblocks * rows;
-  a.xblocks = static_cast<uint32_t>(xblocks);
-  a.rows = static_cast<uint32_t>(rows);
+  // one thread per cell, blocks of 256 consecutive cells (lbm_adjoint_kernel)
+  const uint64_t n = static_cast<uint64_t>(params->nx) * params->ny * params->nz;
+  const uint64_t nblocks = (n + 255) / 256;
   a.div_nx = tlbm::make_divisor(static_cast<uint32_t>(params->nx));
   a.div_ny = tlbm::make_divisor(static_cast<uint32_t>(params->ny));
-  if (nblocks * bx > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
-  const dim3 grid(static_cast<uint32_t>(nblocks)), block(bx);
+  if (nblocks * 256 > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
+  const dim3 grid(static_cast<uint32_t>(nblocks)), block(256);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (final_links && linear)
-    hipLaunchKernelGGL((tlbm::lbm_adjoint_kernel<true, true>), grid, block, 0, st, a);
-  else if (final_links)
-    hipLaunchKernelGGL((tlbm::lbm_adjoint_kernel<true, false>), grid, block, 0, st, a);
-  else if (linear)
-    hipLaunchKernelGGL((tlbm::lbm_adjoint_kernel<false, true>), grid, block, 0, st, a);
+  if (final_links)
+    hipLaunchKernelGGL(tlbm::lbm_adjoint_kernel<true>, grid, block, 0, st, a);
   else
-    hipLaunchKernelGGL((tlbm::lbm_adjoint_kernel<false, false>), grid, block, 0, st, a);
+    hipLaunchKernelGGL(tlbm::lbm_adjoint_kernel<false>, grid, block, 0, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "adjoint launch");
   return TLBM_OK;

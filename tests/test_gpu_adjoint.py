@@ -41,7 +41,7 @@ def _random_case(dims, seed):
     return phi, f0
 
 
-# rows of 64 cells take the row mapping, the others the linear one
+# rows of 64 cells (waves within a row) and shorter ones (waves straddle rows)
 @pytest.mark.parametrize("dims,steps,window,k", [((12, 9, 8), 24, 10, 5), ((7, 5, 6), 13, 13, 4),
                                                  ((33, 4, 3), 9, 3, 100), ((64, 4, 3), 9, 4, 5)])
 def test_adjoint_matches_fp64_autograd_on_every_cell(dims, steps, window, k):
