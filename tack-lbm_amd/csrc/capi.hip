@@ -588,14 +588,11 @@ int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, fl
   if (int rc = check_params(params, d.obstacle)) return rc;
   tlbm::StepArgs a = make_args(*params, d.obstacle);
   a.fout = f;
-  const uint64_t xblocks = (params->nx + 255) / 256;
-  const uint64_t rows = static_cast<uint64_t>(params->ny) * params->nz;
-  if (xblocks * rows * 256 > 0xFFFFFFFFull)
-    return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
-  a.xblocks = static_cast<uint32_t>(xblocks);
-  a.rows = static_cast<uint32_t>(rows);
+  const uint64_t n = static_cast<uint64_t>(params->nx) * params->ny * params->nz;
+  const uint64_t nblocks = (n + 255) / 256;
+  if (nblocks * 256 > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
   const dim3 block(256);
-  const dim3 grid(static_cast<uint32_t>(xblocks * rows));
+  const dim3 grid(static_cast<uint32_t>(nblocks));
   hipError_t e = kInit[d.storage][d.obstacle](a, seed_amp, noise_amp, noise_seed, grid, block,
                                               static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "lbm_init launch");
@@ -808,10 +805,8 @@ int tlbm_batch_init(tlbm_batch* b, void* f, float seed_amp, float noise_amp, uin
     const tlbm_params& p = b->params[i];
     tlbm::StepArgs a = b->dom[i].a;
     a.fout = static_cast<char*>(f) + b->dom[i].elem_off * (b->spec.storage == TLBM_F32 ? 4 : 2);
-    const uint64_t xblocks = (p.nx + 255) / 256;
-    a.xblocks = static_cast<uint32_t>(xblocks);
-    a.rows = static_cast<uint32_t>(static_cast<uint64_t>(p.ny) * p.nz);
-    const dim3 grid(static_cast<uint32_t>(xblocks * a.rows));
+    const uint64_t n = static_cast<uint64_t>(p.nx) * p.ny * p.nz;
+    const dim3 grid(static_cast<uint32_t>((n + 255) / 256));
     hipError_t e = kInit[b->spec.storage][b->spec.obstacle](a, seed_amp, noise_amp, noise_seed,
                                                            grid, dim3(256),
                                                            static_cast<hipStream_t>(stream));

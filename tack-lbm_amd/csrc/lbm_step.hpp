@@ -150,7 +150,6 @@ struct StepArgs {
   tlbm_fields fld;
   double* partials;          // report: per-block FP64 partial sums [TLBM_NSUMS][nblocks]
   uint32_t nblocks;
-  uint32_t xblocks, rows;    // row-mapped launches (lbm_init_kernel): blocks per x-row, rows
   Divisor div_nx, div_ny;    // cell index -> (x, y, z)
   // temperature variant (lbm.py:457-492)
   const float* g_in;
@@ -478,30 +477,6 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
 }
 
 // ---- XCD-aware block -> tile mapping -----------------------------------------------------------
-// Row-mapped kernels (lbm_init_kernel; the adjoint on rows that are a multiple of 64 cells).
-// The grid is one-dimensional: xblocks blocks per x-row, rows = ny * nz rows.  Workgroups are
-// dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8), and each XCD has its own
-// L2.  The x-blocks of one row share a cache line at every block edge (the x +- 1 pulls), so
-// the blocks of a row are placed on one XCD: in each group of 8 * xblocks consecutive block
-// ids, XCD j holds all x-blocks of the group's j-th row.  Rows past the last whole group
-// (rows % 8 of them) take the plain order.
-constexpr uint32_t kXcds = 8;
-
-__device__ __forceinline__ void tile_of_block(uint32_t b, uint32_t xblocks, uint32_t rows,
-                                              uint32_t& xb, uint32_t& row) {
-  const uint32_t group = kXcds * xblocks;
-  const uint32_t whole = (rows / kXcds) * group;
-  if (b < whole) {
-    const uint32_t g = b / group, r = b % group;
-    row = g * kXcds + r % kXcds;
-    xb = r / kXcds;
-  } else {
-    const uint32_t r = b - whole;
-    row = (rows / kXcds) * kXcds + r / xblocks;
-    xb = r % xblocks;
-  }
-}
-
 // ---- the fused step of one cell ------------------------------------------------------------------
 // Pull, bounce-back, collision, sponge and store for cell (x, y, z) of the lattice described by
 // `a`, reading `fin` and writing `fout` (plane stride a.ns); x, y and z are per lane.  REPORT: writes the 13 fields at `fidx` (when F.ux is not null) and returns the 14 FP64
@@ -888,13 +863,11 @@ __global__ void __launch_bounds__(256) lbm_init_kernel(const StepArgs a, float s
   using St = StorageT<STORAGE>;
   using T = typename St::T;
   T* __restrict__ f = static_cast<T*>(a.fout);
-  uint32_t xb, row;
-  tile_of_block(blockIdx.x, a.xblocks, a.rows, xb, row);
-  const int x = static_cast<int>(xb * blockDim.x + threadIdx.x);
-  const int y = static_cast<int>(row % static_cast<uint32_t>(a.ny));
-  const int z = static_cast<int>(row / static_cast<uint32_t>(a.ny));
-  if (x >= a.nx) return;
-  const uint32_t idx = (static_cast<uint32_t>(z) * a.ny + y) * static_cast<uint32_t>(a.nx) + x;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<uint32_t>(a.nx) * a.ny * static_cast<uint32_t>(a.nz)) return;
+  uint32_t xr, yr, zr;
+  cell_coords(idx, a.nx, a.ny, a.div_nx, a.div_ny, xr, yr, zr);
+  const int x = static_cast<int>(xr), y = static_cast<int>(yr), z = static_cast<int>(zr);
   const float fluid = 1.0f - Obs::solid(a, x, y, z);
   const float two_pi = 6.2831853f;
   float ux = a.u0 * fluid;
