@@ -125,7 +125,7 @@ inline Divisor make_divisor(uint32_t d) {
   return Divisor{static_cast<uint32_t>(m), l < 1 ? l : 1u, l > 1 ? l - 1 : 0u};
 }
 
-__device__ __forceinline__ uint32_t divide(uint32_t n, const Divisor& d) {
+__host__ __device__ __forceinline__ uint32_t divide(uint32_t n, const Divisor& d) {
   const uint32_t t = static_cast<uint32_t>((static_cast<uint64_t>(d.m) * n) >> 32);
   return (t + ((n - t) >> d.sh1)) >> d.sh2;
 }
