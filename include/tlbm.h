@@ -66,7 +66,8 @@ typedef struct {
 /* Runtime arguments of one step (the reference's plugin instance attributes). */
 typedef struct {
   int32_t nx, ny, nz;      /* D3Q19(nx, ny, nz), lbm.py:42-44                           */
-  int32_t reserved0;
+  int32_t block_x;         /* threads per block of the step: 64, 128, 192 or 256; 0: the
+                              handle's default (tlbm_set_launch, else 256)                */
   int64_t qstride;         /* elements between population planes, >= nx*ny*nz           */
   float omega;             /* BGK: 1 / (3 nu + 1/2), lbm.py:63                          */
   float tau0;              /* TRT / LES: 3 nu + 1/2, lbm.py:110, 131                    */

@@ -657,19 +657,25 @@ class OracleSolver:
         self.flow = Flow(u0, 0.75 * nx if sponge_start is None else sponge_start,
                          sponge_strength, 0.0, side)
         self.geo = make_obstacle(obstacle, nx, ny, nz, diameter, center)
-        self.nu = u0 * diameter / float(reynolds)
-        kw = {}
-        if cs is not None and collision in ("trt", "les"):
-            kw["cs"] = cs
-        if magic is not None and collision == "trt":
-            kw["magic"] = magic
-        self.phys = COLLISIONS[collision](self.nu, **kw)
+        self._cs, self._magic = cs, magic
+        self.set_physics(collision, reynolds)
         self.fields = new_fields(self.n, self.dtype)
         self.f = init(self.dims, self.geo, self.flow, seed_amp, noise_amp, noise_seed, self.dtype)
         self.step_count = 0
         if construct_report:                                      # lbm.py:582-584
             self.step(report=True)
             self.step_count = 0
+
+    def set_physics(self, collision, reynolds):
+        """lbm.py:613-616: nu = u0 D / Re and the named collision model."""
+        self.collision_name, self.reynolds = collision, float(reynolds)
+        self.nu = self.u0 * self.diameter / self.reynolds
+        kw = {}
+        if self._cs is not None and collision in ("trt", "les"):
+            kw["cs"] = self._cs
+        if self._magic is not None and collision == "trt":
+            kw["magic"] = self._magic
+        self.phys = COLLISIONS[collision](self.nu, **kw)
 
     def set_actuation(self, omega_rot):
         self.flow.omega_rot = float(omega_rot)

@@ -107,56 +107,107 @@ bool decode(int h, Decoded& d) {
 // ---- report workspace, one per (device, stream), grow-only ----------------------------------
 // A report step's block partials live here until its reduction (same stream) has read them, so
 // report steps queued on different streams each need their own buffer; steps on one stream are
-// ordered by the stream.
-struct Workspace {
+// ordered by the stream.  A graph captured from report steps holds the buffer's address in its
+// kernel nodes, so each buffer counts the live graphs that use it: a buffer the stream outgrows
+// (or whose stream is destroyed) is freed only once no graph refers to it any more.
+struct WsBuffer {
   double* ptr = nullptr;
   size_t bytes = 0;
+  int device = 0;
+  int graphs = 0;        // live tlbm_graph objects whose kernels use this buffer
+  bool retired = false;  // no longer a stream's current buffer: free when graphs reaches 0
 };
 std::mutex g_ws_mutex;
-std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+std::map<std::pair<int, hipStream_t>, WsBuffer*> g_ws;
+// buffers handed to report steps recorded while a stream was capturing, per stream
+std::map<hipStream_t, std::vector<WsBuffer*>> g_ws_captured;
+
+// Frees a retired buffer that no graph uses (caller holds g_ws_mutex).
+void ws_free_if_unused(WsBuffer* b) {
+  if (!b->retired || b->graphs > 0) return;
+  int current = 0;
+  const bool restore = hipGetDevice(&current) == hipSuccess;
+  if (b->ptr && hipSetDevice(b->device) == hipSuccess) {
+    (void)hipDeviceSynchronize();  // queued work on any stream may still use it
+    (void)hipFree(b->ptr);
+  }
+  if (restore) (void)hipSetDevice(current);
+  delete b;
+}
 
 int workspace(size_t bytes, hipStream_t stream, double** out) {
   int dev = 0;
   TLBM_HIP(hipGetDevice(&dev));
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  TLBM_HIP(hipStreamIsCapturing(stream, &cs));
+  const bool capturing = cs != hipStreamCaptureStatusNone;
   std::lock_guard<std::mutex> lock(g_ws_mutex);
-  Workspace& w = g_ws[{dev, stream}];
-  if (w.bytes < bytes) {
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    TLBM_HIP(hipStreamIsCapturing(stream, &cs));
-    if (cs != hipStreamCaptureStatusNone)
+  WsBuffer*& w = g_ws[{dev, stream}];
+  if (!w || w->bytes < bytes) {
+    if (capturing)
       return fail(TLBM_ERR_INVALID,
                   "report workspace of this stream must grow to %zu bytes, which cannot happen "
                   "under graph capture: run one report step eagerly first", bytes);
-    if (w.ptr) {
-      TLBM_HIP(hipDeviceSynchronize());
-      TLBM_HIP(hipFree(w.ptr));
-      w.ptr = nullptr;
-      w.bytes = 0;
+    if (w) {
+      w->retired = true;
+      ws_free_if_unused(w);
+      w = nullptr;
     }
     void* p = nullptr;
-    TLBM_HIP(hipMalloc(&p, bytes));
-    w.ptr = static_cast<double*>(p);
-    w.bytes = bytes;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc (report workspace)");
+    w = new WsBuffer();
+    w->ptr = static_cast<double*>(p);
+    w->bytes = bytes;
+    w->device = dev;
   }
-  *out = w.ptr;
+  if (capturing) {
+    std::vector<WsBuffer*>& used = g_ws_captured[stream];
+    if (std::find(used.begin(), used.end(), w) == used.end()) used.push_back(w);
+  }
+  *out = w->ptr;
   return TLBM_OK;
 }
 
-// Frees the report workspaces of a stream (on every device) once its work has finished.
+// The buffers a capture on `stream` used, now referenced by a graph (or dropped with it).
+std::vector<WsBuffer*> take_captured(hipStream_t stream, bool keep) {
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  std::vector<WsBuffer*> used;
+  auto it = g_ws_captured.find(stream);
+  if (it != g_ws_captured.end()) {
+    used.swap(it->second);
+    g_ws_captured.erase(it);
+  }
+  if (keep)
+    for (WsBuffer* b : used) ++b->graphs;
+  return keep ? used : std::vector<WsBuffer*>();
+}
+
+void release_graph_buffers(std::vector<WsBuffer*>& used) {
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  for (WsBuffer* b : used) {
+    --b->graphs;
+    ws_free_if_unused(b);
+  }
+  used.clear();
+}
+
+// Retires the report workspaces of a stream (on every device) once its work has finished;
+// a buffer that graphs still use is freed with the last of them.
 void release_workspaces(hipStream_t stream) {
   std::lock_guard<std::mutex> lock(g_ws_mutex);
-  int current = 0;
-  const bool restore = hipGetDevice(&current) == hipSuccess;
   for (auto it = g_ws.begin(); it != g_ws.end();) {
     if (it->first.second == stream) {
-      if (it->second.ptr && hipSetDevice(it->first.first) == hipSuccess)
-        (void)hipFree(it->second.ptr);
+      if (it->second) {
+        it->second->retired = true;
+        ws_free_if_unused(it->second);
+      }
       it = g_ws.erase(it);
     } else {
       ++it;
     }
   }
-  if (restore) (void)hipSetDevice(current);
+  g_ws_captured.erase(stream);
 }
 
 // Sums partials[k][0..nblocks) in a fixed order (block k sums row k): deterministic FP64 totals.
@@ -222,12 +273,19 @@ bool stream_stores(uint64_t elements, int storage) {
   return 2 * elements * (storage == TLBM_F32 ? 4 : 2) > kStreamStoreBytes;
 }
 
-tlbm::StepArgs make_args(const tlbm_params& p, int obstacle) {
+// elem_bytes: bytes per stored population (4 for FP32, 2 for FP16/BF16).
+tlbm::StepArgs make_args(const tlbm_params& p, int obstacle, uint32_t elem_bytes) {
   tlbm::StepArgs a{};
   a.ns = static_cast<uint64_t>(p.qstride);
   a.nx = p.nx;
   a.ny = p.ny;
   a.nz = p.nz;
+  // Buffer-resource addressing (lbm_step.hpp) needs every plane offset plus every lane offset
+  // below 2^32 bytes: the last plane's start (18 qstride) plus a cell index and one row.
+  const uint64_t cells = static_cast<uint64_t>(p.nx) * p.ny * p.nz;
+  const uint64_t span = 19ull * static_cast<uint64_t>(p.qstride) + cells +
+                        static_cast<uint64_t>(p.nx);
+  a.off32 = span * elem_bytes < (1ull << 32) ? 1u : 0u;
   a.div_nx = tlbm::make_divisor(static_cast<uint32_t>(p.nx));
   a.div_ny = tlbm::make_divisor(static_cast<uint32_t>(p.ny));
   a.omega = p.omega;
@@ -337,6 +395,7 @@ struct tlbm_batch {
 struct tlbm_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<WsBuffer*> workspaces;  // report workspaces the graph's kernels write
 };
 
 namespace {
@@ -362,7 +421,8 @@ int batch_build(tlbm_batch* b) {
     const tlbm_params& p = b->params[d];
     const uint64_t n = static_cast<uint64_t>(p.nx) * p.ny * p.nz;
     tlbm::BatchDomain& D = b->dom[d];
-    D.a = make_args(p, b->spec.obstacle);  // a.ns = the domain's plane stride (>= n)
+    // a.ns = the domain's plane stride (>= n)
+    D.a = make_args(p, b->spec.obstacle, b->spec.storage == TLBM_F32 ? 4u : 2u);
     D.elem_off = elem;
     D.cell_off = static_cast<uint32_t>(cell);
     D.chunk_off = static_cast<uint32_t>(chunk);
@@ -586,7 +646,7 @@ int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, fl
   if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
   if (!f) return fail(TLBM_ERR_INVALID, "f is NULL");
   if (int rc = check_params(params, d.obstacle)) return rc;
-  tlbm::StepArgs a = make_args(*params, d.obstacle);
+  tlbm::StepArgs a = make_args(*params, d.obstacle, d.storage == TLBM_F32 ? 4u : 2u);
   a.fout = f;
   const uint64_t n = static_cast<uint64_t>(params->nx) * params->ny * params->nz;
   const uint64_t nblocks = (n + 255) / 256;
@@ -610,7 +670,7 @@ int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const
   if (!f_in || !f_out) return fail(TLBM_ERR_INVALID, "f_in or f_out is NULL");
   if (f_in == f_out) return fail(TLBM_ERR_INVALID, "f_in and f_out must be distinct buffers");
   if (int rc = check_params(params, d.obstacle)) return rc;
-  tlbm::StepArgs a = make_args(*params, d.obstacle);
+  tlbm::StepArgs a = make_args(*params, d.obstacle, d.storage == TLBM_F32 ? 4u : 2u);
   a.fin = f_in;
   a.fout = f_out;
   // the temperature step adds 7 FP32 planes (g) to the 19 population planes
@@ -633,8 +693,12 @@ int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const
     a.nu_mol = heat->nu;
     a.pr_t = heat->pr_t;
   }
-  // one thread per cell, blocks of consecutive cells in index order (lbm_step_kernel)
-  const int bx = g_block_x[handle] ? g_block_x[handle] : 256;
+  // one thread per cell, blocks of consecutive cells in index order (lbm_step_kernel); the
+  // block size of this launch, else the handle's (tlbm_set_launch), else 256
+  if (params->block_x < 0 || params->block_x > 256 || params->block_x % 64 != 0)
+    return fail(TLBM_ERR_INVALID, "params->block_x must be 0, 64, 128, 192 or 256 (got %d)",
+                params->block_x);
+  const int bx = params->block_x ? params->block_x : (g_block_x[handle] ? g_block_x[handle] : 256);
   const uint64_t n = static_cast<uint64_t>(params->nx) * params->ny * params->nz;
   const uint64_t nblocks64 = (n + bx - 1) / bx;
   if (nblocks64 * bx > 0xFFFFFFFFull)
@@ -889,6 +953,7 @@ int tlbm_batch_graph(tlbm_batch* b, void* f_a, void* f_b, int steps, int report_
 
 int tlbm_capture_begin(void* stream) {
   if (!stream) return fail(TLBM_ERR_INVALID, "graph capture needs a created (non-null) stream");
+  take_captured(static_cast<hipStream_t>(stream), false);  // forget an abandoned capture's list
   TLBM_HIP(hipStreamBeginCapture(static_cast<hipStream_t>(stream),
                                  hipStreamCaptureModeThreadLocal));
   return TLBM_OK;
@@ -897,25 +962,30 @@ int tlbm_capture_begin(void* stream) {
 int tlbm_capture_end(void* stream, tlbm_graph** out) {
   if (!stream) return fail(TLBM_ERR_INVALID, "graph capture needs a created (non-null) stream");
   if (out) *out = nullptr;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
   hipGraph_t graph = nullptr;
-  hipError_t e = hipStreamEndCapture(static_cast<hipStream_t>(stream), &graph);
+  hipError_t e = hipStreamEndCapture(st, &graph);
   if (e != hipSuccess) {
+    take_captured(st, false);
     if (graph) (void)hipGraphDestroy(graph);
     return hip_fail(e, "hipStreamEndCapture");
   }
   if (!out) {
+    take_captured(st, false);
     (void)hipGraphDestroy(graph);
     return fail(TLBM_ERR_INVALID, "out is NULL (the capture was discarded)");
   }
   hipGraphExec_t exec = nullptr;
   e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
+    take_captured(st, false);
     (void)hipGraphDestroy(graph);
     return hip_fail(e, "hipGraphInstantiate");
   }
   tlbm_graph* g = new tlbm_graph();
   g->graph = graph;
   g->exec = exec;
+  g->workspaces = take_captured(st, true);  // kept alive until tlbm_graph_destroy
   *out = g;
   return TLBM_OK;
 }
@@ -930,6 +1000,7 @@ int tlbm_graph_destroy(tlbm_graph* g) {
   if (!g) return TLBM_OK;
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  release_graph_buffers(g->workspaces);
   delete g;
   return TLBM_OK;
 }

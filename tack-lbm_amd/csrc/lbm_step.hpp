@@ -109,6 +109,40 @@ struct StoreBF16 {
   }
 };
 
+// ---- population buffers as buffer resources ------------------------------------------------
+// A raw buffer resource over a population buffer: address = base + scalar offset (the plane)
+// + vector offset (the lane's cell), 32-bit each, so 19 loads or stores of a cell share one
+// lane offset.  Used when the buffer spans less than 2^32 bytes (StepArgs::off32); the
+// descriptor's third word is the gfx9 raw-buffer format (32-bit data, no swizzle).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
+}
+
+template <class T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(
+                                     r, static_cast<int>(voff), static_cast<int>(soff), 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(
+                                     r, static_cast<int>(voff), static_cast<int>(soff), 0));
+  }
+}
+
+// NT: the non-temporal cache policy (the "nt" bit, 2 in the cache-policy operand)
+template <class T, bool NT>
+__device__ __forceinline__ void buffer_store(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                             uint32_t soff) {
+  constexpr int pol = NT ? 2 : 0;
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r,
+                                          static_cast<int>(voff), static_cast<int>(soff), pol);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r,
+                                          static_cast<int>(voff), static_cast<int>(soff), pol);
+  }
+}
+
 // ---- unsigned division by a divisor fixed for the launch ----------------------------------
 // Granlund and Montgomery, "Division by invariant integers using multiplication" (PLDI 1994),
 // section 4: with l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, sh1 = min(l, 1) and
@@ -138,6 +172,9 @@ struct StepArgs {
   int32_t nx, ny, nz;
   int32_t wrap;              // bit 0/1/2: the obstacle reaches across the x/y/z periodic faces
   uint32_t stream_stores;    // 1: launch the non-temporal-store (NT) quiet kernel (host side)
+  uint32_t off32;            // 1: the whole population buffer lies within 2^32 bytes of its
+                             // start, so it is addressed as one buffer resource with 32-bit
+                             // plane (scalar) and lane (vector) offsets
   float omega;               // BGK
   float tau0, smag, magic;   // smag = 25.456 cs^2 (lbm.py:93)
   float o[8];                // obstacle parameters
@@ -612,17 +649,39 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   // No branch between the 19 loads, so that all of them are in flight together: ceiling links
   // of the lid read their in-bounds (clamped) row and are overwritten below.
   float f[kQ];
-  static_for<0, kQ>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-    int qs = q;
-    if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = mir_lo ? miry(q) : q;
-    if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = mir_hi ? miry(q) : q;
-    const T* __restrict__ row = fin + static_cast<uint64_t>(qs) * a.ns +
-                                static_cast<uint32_t>(zsrc[1 + cz]) * plane +
-                                static_cast<uint32_t>(ysrc[1 + cy]) * snx;
-    f[q] = St::value(row[xsrc[1 + cx]]);
-  });
+  // Interior waves: when no lane of the wave lies on a y or z face, the source of population q
+  // is idx - (cx + cy nx + cz nx ny) for every lane, except the x wrap of one lane at each row
+  // end.  So each plane gets one wave-uniform base pointer (scalar registers) and the 19 loads
+  // share three 32-bit lane offsets: a handful of vector instructions instead of a 64-bit
+  // address per load and lane, which made the 16-bit kernels instruction-bound.  Waves that
+  // touch a y or z face (mirror, lid or wrap per lane) take the general per-lane path.
+  const bool inner = y >= 1 && y < ny - 1 && z >= 1 && z < nz - 1;
+  if (a.off32 && __builtin_amdgcn_ballot_w64(!inner) == 0) {
+    constexpr uint32_t B = sizeof(T);
+    const auto rin = plane_rsrc(fin);
+    const uint32_t v0 = idx * B;
+    const uint32_t vp = (idx + (x == 0 ? snx : 0u)) * B;       // c_x = +1: x - 1, wrapped at 0
+    const uint32_t vm = (idx - (x == nx - 1 ? snx : 0u)) * B;  // c_x = -1: x + 1, wrapped
+    static_for<0, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+      const int64_t shift = cx + cy * static_cast<int64_t>(nx) + cz * static_cast<int64_t>(plane);
+      const uint32_t soff = static_cast<uint32_t>((static_cast<int64_t>(q) * a.ns - shift) * B);
+      f[q] = St::value(buffer_load<T>(rin, cx == 1 ? vp : (cx == -1 ? vm : v0), soff));
+    });
+  } else {
+    static_for<0, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+      int qs = q;
+      if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = mir_lo ? miry(q) : q;
+      if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = mir_hi ? miry(q) : q;
+      const T* __restrict__ row = fin + static_cast<uint64_t>(qs) * a.ns +
+                                  static_cast<uint32_t>(zsrc[1 + cz]) * plane +
+                                  static_cast<uint32_t>(ysrc[1 + cy]) * snx;
+      f[q] = St::value(row[xsrc[1 + cx]]);
+    });
+  }
 
   // ---- moving lid: f_q = f_opp(x) + 6 w_q c_q . u_lid on ceiling links ----
   if constexpr (SIDE == TLBM_LID_Y) {
@@ -656,7 +715,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
     T own[kQ];
     static_for<1, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      own[q] = fin[static_cast<uint64_t>(opp(q)) * a.ns + idx];
+      own[q] = (fin + static_cast<uint64_t>(opp(q)) * a.ns)[idx];
     });
     static_for<1, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
@@ -698,12 +757,23 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
     // NT: non-temporal stores, for lattices whose two population buffers are far larger than
     // the Infinity Cache (measured on MI355X: 1.6% faster at FP32, 3.4% at FP16, 512x256x256);
     // below that size the next step's loads hit the cache lines plain stores leave there.
-    static_for<0, kQ>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      T* dst = &fout[static_cast<uint64_t>(q) * a.ns + idx];
-      if constexpr (NT) __builtin_nontemporal_store(St::encode(f[q]), dst);
-      else *dst = St::encode(f[q]);
-    });
+    if (a.off32) {
+      // one lane offset for all 19 planes, the plane offsets in scalar registers
+      constexpr uint32_t B = sizeof(T);
+      const auto rout = plane_rsrc(fout);
+      static_for<0, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        buffer_store<T, NT>(St::encode(f[q]), rout, idx * B,
+                            static_cast<uint32_t>(static_cast<uint64_t>(q) * a.ns * B));
+      });
+    } else {
+      static_for<0, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        T* const dst = (fout + static_cast<uint64_t>(q) * a.ns) + idx;
+        if constexpr (NT) __builtin_nontemporal_store(St::encode(f[q]), dst);
+        else *dst = St::encode(f[q]);
+      });
+    }
   }
   if constexpr (HEAT) heat_cell<OBS, REPORT, NT>(a, x, y, z, idx, here, ux, uy, uz, ex, active);
 

@@ -28,7 +28,7 @@ class Spec(ctypes.Structure):
 
 class Params(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
-                ("reserved0", ctypes.c_int32), ("qstride", ctypes.c_int64),
+                ("block_x", ctypes.c_int32), ("qstride", ctypes.c_int64),
                 ("omega", ctypes.c_float), ("tau0", ctypes.c_float), ("cs", ctypes.c_float),
                 ("magic", ctypes.c_float), ("obs", ctypes.c_float * 8), ("u0", ctypes.c_float),
                 ("sponge_start", ctypes.c_float), ("sponge_strength", ctypes.c_float),

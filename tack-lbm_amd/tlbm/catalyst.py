@@ -16,6 +16,9 @@ from __future__ import annotations
 import numpy as np
 
 COLLISION_CODES = {"bgk": 0, "trt": 1, "les": 2}  # steering.py:7
+COLLISION_NAMES = {code: name for name, code in COLLISION_CODES.items()}
+# the scalar properties a steering client may send (steering.py:141)
+STEERABLE = ("reynolds", "collision", "power", "gust", "stiffness", "rotation", "capture_stride")
 # run_lbm.py:25-26
 MONITOR_NAMES = ("continuity", "momentum_x", "momentum_y", "momentum_z", "energy", "drag", "lift",
                  "reynolds", "collision", "rotation", "derived_error", "time")
@@ -52,6 +55,83 @@ def rotation_rate(solver):
     return getattr(solver, "omega_rot", 0.0) / solver.u0
 
 
+def set_rotation_rate(solver, alpha):
+    """run_lbm.py:33-34: set the cylinder surface speed to alpha times the inflow speed."""
+    solver.set_actuation(float(alpha) * solver.u0)
+
+
+class RotationSchedule:
+    """run_lbm.py:206-213: "step:alpha,step:alpha,..." events, applied once the solver has
+    taken at least that many steps (each event once, in step order)."""
+
+    def __init__(self, spec=""):
+        events = []
+        for item in spec.split(","):
+            item = item.strip()
+            if not item:
+                continue
+            step, sep, alpha = item.partition(":")
+            if not sep:
+                raise ValueError(f"a rotation event is STEP:ALPHA, not {item!r}")
+            events.append((int(step), float(alpha)))
+        self.events = sorted(events)
+
+    def apply(self, solver):
+        """Apply every due event; returns the (step, alpha) pairs applied."""
+        done = []
+        while self.events and solver.step_count >= self.events[0][0]:
+            event = self.events.pop(0)
+            set_rotation_rate(solver, event[1])
+            done.append(event)
+        return done
+
+
+def _text(codes):
+    """steering.py:116-121: text sent as an array of UTF-8 byte values (0 alone: empty)."""
+    values = np.atleast_1d(np.asarray(codes)).reshape(-1)
+    return bytes(int(c) for c in values if 0 < int(c) < 256).decode("utf-8", "replace").strip()
+
+
+def _first(value):
+    return float(np.atleast_1d(np.asarray(value, dtype=np.float64)).reshape(-1)[0])
+
+
+def _lookup(tree, path):
+    node = tree
+    for part in path.split("/"):
+        if not isinstance(node, dict) or part not in node:
+            return None
+        node = node[part]
+    return node
+
+
+def read_steering(results):
+    """steering.py:124-147 over a results tree (nested dicts keyed by path component, as
+    ``CatalystChannels.execute`` builds them): the derived-field request text and the steered
+    scalars of every part under ``catalyst/derived/data`` and ``catalyst/steerable/data``, each
+    from ``state/fields/<key>`` or ``fields/<key>/values``.  Missing values are left out."""
+    values = {}
+    for mesh in ("catalyst/derived/data", "catalyst/steerable/data"):
+        data = _lookup(results, mesh)
+        if not isinstance(data, dict):
+            continue
+        for part in data.values():
+            for path in ("state/fields/derived", "fields/derived/values"):
+                v = _lookup(part, path)
+                if v is not None:
+                    values["derived"] = _text(v)
+    data = _lookup(results, "catalyst/steerable/data")
+    if not isinstance(data, dict):
+        return values
+    for part in data.values():
+        for key in STEERABLE:
+            for path in (f"state/fields/{key}", f"fields/{key}/values"):
+                v = _lookup(part, path)
+                if v is not None:
+                    values[key] = _first(v)
+    return values
+
+
 class CatalystChannels:
     """run_lbm.py:38-121 without the Catalyst calls.  ``scales``: optional SI output,
     {"length": metres per cell, quantity: (factor, offset)} (run_lbm.py:51-52)."""
@@ -64,6 +144,32 @@ class CatalystChannels:
         self.scales = scales
         self.monitor_history = monitor_history
         self.history = {name: [] for name in ("iteration", *self.monitor_names)}
+        # the values a steering client last set (run_lbm.py:55-56)
+        self.steered = {"reynolds": solver.reynolds,
+                        "collision": float(COLLISION_CODES[solver.collision_name]),
+                        "rotation": rotation_rate(solver), "capture_stride": 0.0}
+
+    def steer(self, values):
+        """run_lbm.py:123-144: apply steered values (``read_steering``'s dict).  A changed
+        rotation rate sets the cylinder's surface speed; a changed Reynolds number or collision
+        model calls ``set_physics`` (a collision change selects another kernel).  Returns the
+        values that changed."""
+        values = dict(values)
+        text = values.pop("derived", None)
+        if text is not None and self.derived is not None and hasattr(self.derived, "steer"):
+            self.derived.steer(text)
+        s = self.solver
+        changed = {k: float(v) for k, v in values.items()
+                   if abs(float(v) - self.steered.get(k, float(v))) > 1e-9}
+        if not changed:
+            return changed
+        self.steered.update(changed)
+        if "rotation" in changed:
+            set_rotation_rate(s, self.steered["rotation"])
+        if changed.keys() - {"rotation", "capture_stride"}:
+            collision = COLLISION_NAMES.get(int(self.steered["collision"]), s.collision_name)
+            s.set_physics(collision, self.steered["reynolds"])
+        return changed
 
     def _values(self, key, quantity):
         arr = self.solver.output[key].numpy

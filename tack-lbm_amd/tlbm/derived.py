@@ -13,6 +13,7 @@ this build: they raise ``NotImplementedError``.
 from __future__ import annotations
 
 import ctypes
+import re
 
 import numpy as np
 
@@ -25,20 +26,22 @@ DEFAULTS = {
 }
 
 
+_SPEC = re.compile(r"([^=]*)=(.*?)(?:@(cells|points))?", re.S)
+
+
 def parse(specs):
-    """'name=expression' or 'name=expression@cells' (default @points) into {name: (expr, at)}
-    (derived.py:32-44)."""
-    out = {}
-    for spec in specs:
-        name, _, expr = spec.partition("=")
+    """Derived-field specifications NAME=EXPRESSION[@cells|@points] (derived.py:32-44) into
+    {name: (expression, location)}: the name is what precedes the first '=', a trailing
+    '@cells' or '@points' picks the location (points when absent), whitespace around the
+    name and expression is dropped."""
+    fields = {}
+    for text in specs:
+        m = _SPEC.fullmatch(text)
+        name, expr = (m.group(1).strip(), m.group(2).strip()) if m else ("", "")
         if not name or not expr:
-            raise ValueError(f"a derived field is NAME=EXPRESSION[@cells|@points], not {spec!r}")
-        at = "points"
-        body, sep, where = expr.rpartition("@")
-        if sep and where in ("cells", "points"):
-            expr, at = body, where
-        out[name.strip()] = (expr.strip(), at)
-    return out
+            raise ValueError(f"a derived field is NAME=EXPRESSION[@cells|@points], not {text!r}")
+        fields[name] = (expr, m.group(3) or "points")
+    return fields
 
 
 def _kind(expr, at):

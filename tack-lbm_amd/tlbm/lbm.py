@@ -32,6 +32,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import weakref
 
 import numpy as np
 
@@ -69,16 +70,20 @@ class DeviceArray:
     f_q - w_q in binary16 / bfloat16 (raw uint16 on the host side for bf16).
     """
 
-    def __init__(self, size, storage="f32"):
+    def __init__(self, size, storage="f32", ptr=None):
         if storage not in STORAGES:
             raise ValueError(f"storage must be one of {sorted(STORAGES)}, got {storage!r}")
         self.size = int(size)
         self.storage = storage
         self.dtype = np.dtype(_RAW_DTYPE[storage])
         self.nbytes = self.size * self.dtype.itemsize
-        p = ctypes.c_void_p()
-        check(lib().tlbm_alloc_device(self.nbytes, ctypes.byref(p)), "device allocation")
-        self.ptr = p.value
+        # ptr given: a view of memory someone else owns (ManagedArray.tack); never freed here
+        self._owned = ptr is None
+        if ptr is None:
+            p = ctypes.c_void_p()
+            check(lib().tlbm_alloc_device(self.nbytes, ctypes.byref(p)), "device allocation")
+            ptr = p.value
+        self.ptr = ptr
 
     def numpy(self):
         """A host copy of the raw contents."""
@@ -99,7 +104,8 @@ class DeviceArray:
 
     def free(self):
         if getattr(self, "ptr", None):
-            lib().tlbm_free(self.ptr)
+            if self._owned:
+                lib().tlbm_free(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -107,6 +113,32 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class DeviceField(DeviceArray):
+    """An n-float field in HBM that host code reads as it reads a ``ManagedArray``: ``numpy``
+    is a property that waits for ``stream`` (None: the device) and returns a host copy, so the
+    reference's ``solver.fields["ux"].numpy.min()`` (bench.py:29) and
+    ``heat_fields["q_solid"].numpy[voxels]`` (cht.py:85) read the field after the last step.
+    The copy is read-only: a write to it would not reach the device (write device fields with
+    ``copy_from``)."""
+
+    def __init__(self, size, stream=None):
+        super().__init__(size, "f32")
+        self.stream = stream
+        self.zero_(stream)
+
+    @property
+    def numpy(self):
+        sync(self.stream)
+        out = DeviceArray.numpy(self)
+        out.flags.writeable = False
+        return out
+
+    @property
+    def tack(self):
+        """The device array itself (the kernels take it as they take a ``ManagedArray.tack``)."""
+        return self
 
 
 def array(a, storage="f32"):
@@ -169,13 +201,47 @@ class ManagedArray:
         self.ptr = p.value
         ctype = {np.dtype(np.float32): ctypes.c_float, np.dtype(np.float64): ctypes.c_double}[
             self.dtype]
-        self.numpy = np.ctypeslib.as_array((ctype * self.size).from_address(self.ptr))
+        self._view = np.ctypeslib.as_array((ctype * self.size).from_address(self.ptr))
+
+    @property
+    def numpy(self):
+        """The NumPy view of the allocation (zero-copy; the same array object every time)."""
+        return self._view
+
+    @property
+    def tack(self):
+        """A device-array view of the same memory (lbm.py:539: the Tack view the kernels take),
+        as accepted by ``lbm_step`` and the other kernels; it does not own the memory."""
+        if self.dtype != np.float32:
+            raise TypeError("only float32 managed arrays have a device-array view")
+        view = DeviceArray(self.size, "f32", ptr=self.ptr)
+        view._owner = self  # the managed allocation lives at least as long as its view
+        return view
 
     def free(self):
         if getattr(self, "ptr", None):
-            self.numpy = None
+            self._view = None
             lib().tlbm_free(self.ptr)
             self.ptr = None
+
+
+class OutputArray(ManagedArray):
+    """A managed output array that brings itself up to date when read: ``numpy`` first asks its
+    solver to refresh the output arrays if a report step has run since their last refresh (a
+    copy or coarsening of the device field it mirrors) and waits for the solver's stream.  The
+    array object and its address stay the same, so readers that take it zero-copy (Conduit's
+    ``set_external``, run_lbm.py:88-93) read it again after each report step."""
+
+    def __init__(self, n, refresh):
+        super().__init__(n)
+        self._refresh = refresh  # weakref.WeakMethod of the solver's refresh
+
+    @property
+    def numpy(self):
+        refresh = self._refresh() if self._refresh is not None else None
+        if refresh is not None:
+            refresh()
+        return self._view
 
 
 class HostBuffer:
@@ -507,14 +573,15 @@ def _fields_struct(fields):
 
 
 def lbm_step(lat, phys, geo, flow, diag, f_in, f_out, heat=None, g_in=None, g_out=None,
-             sums=None, stream=None):
+             sums=None, stream=None, block_x=0):
     """lbm.py:397-513: one step from ``f_in`` to ``f_out``.
 
     ``diag`` is ``Quiet``/``Report``; a ``Report`` step writes its 13 fields and, when ``sums``
     (a ``HostBuffer``/``ManagedArray`` of 14 float64) is given, their FP64 totals plus the total
     mass.  With ``heat.enabled == 1`` (a ``Heat``) the temperature populations step from ``g_in``
     to ``g_out`` too (FP32), and a report step writes ``heat.temp`` and adds to
-    ``heat.q_solid``.
+    ``heat.q_solid``.  ``block_x``: threads per block of this launch (0: the specialisation's
+    default).
     """
     if f_in.storage != f_out.storage:
         raise ValueError(f"f_in is {f_in.storage}, f_out is {f_out.storage}")
@@ -527,6 +594,7 @@ def lbm_step(lat, phys, geo, flow, diag, f_in, f_out, heat=None, g_in=None, g_ou
     if _qstride(lat, f_out) != qs:
         raise ValueError("f_in and f_out have different plane strides")
     p = make_params(lat, phys, geo, flow, qs)
+    p.block_x = int(block_x)
     fs = _fields_struct(diag.fields) if report else None
     if not hot:
         check(lib().tlbm_step(h, f_in.ptr, f_out.ptr, ctypes.byref(p),
@@ -589,6 +657,7 @@ class Solver:
         self.u0, self.diameter = u0, diameter
         self.storage = storage
         self.step_count = 0
+        self.block_x = 0
         self._omega_state = ManagedArray(1)
         side = LidY(*lid_u) if lateral == "lid" else SIDES[lateral]()
         self.flow = Flow(u0, 0.75 * nx if sponge_start is None else sponge_start,
@@ -610,7 +679,7 @@ class Solver:
             raise ValueError(f"field_memory must be 'device' or 'managed', got {field_memory!r}")
         self.field_memory = field_memory
         if field_memory == "device":
-            self.fields = {name: zeros((n,)) for name in FIELDS}
+            self.fields = {name: DeviceField(n, stream) for name in FIELDS}
         else:
             self.fields = {name: ManagedArray(n) for name in FIELDS}
         self.quiet = Quiet(self.fields)
@@ -622,9 +691,13 @@ class Solver:
         self.stride = stride
         self.out_dims = tuple((d + stride - 1) // stride for d in (nx, ny, nz))
         outputs = self.OUTPUT + (("temp",) if self.heat_enabled else ())
+        # Output arrays: the fields themselves when they are managed and not coarsened (the
+        # reference's layout, lbm.py:577-580); otherwise managed copies that refresh when read.
+        self._output_stale = False
         if stride > 1 or field_memory == "device":
             m = self.out_dims[0] * self.out_dims[1] * self.out_dims[2]
-            self.output = {name: ManagedArray(m) for name in outputs}
+            refresh = weakref.WeakMethod(self._refresh_output)
+            self.output = {name: OutputArray(m, refresh) for name in outputs}
         else:
             self.output = {name: self.heat_fields.get(name) or self.fields[name]
                            for name in outputs}
@@ -635,13 +708,21 @@ class Solver:
 
     def _setup_heat(self, n):
         """lbm.py:586-600: the temperature fields (n floats each when heat is on, else one)
-        and the D3Q7 population buffers, zero (lbm.py:594)."""
+        and the D3Q7 population buffers, zero (lbm.py:594).
+
+        Also callable on a bare ``Solver.__new__(Solver)`` that has only ``heat_enabled`` set,
+        as trt_ref.py:33-36 does to get ``heat``, ``g[0]`` and ``g[1]`` for a direct
+        ``lbm_step`` call: unset attributes take the reference's layout (managed temperature,
+        planes of n cells)."""
         small = n if self.heat_enabled else 1
-        temp = zeros((small,)) if self.field_memory == "device" else ManagedArray(small)
-        self.heat_fields = {"t_solid": ManagedArray(small), "q_solid": zeros((small,)),
+        stream = getattr(self, "stream", None)
+        managed = getattr(self, "field_memory", "managed") == "managed"
+        temp = ManagedArray(small) if managed else DeviceField(small, stream)
+        # q_solid takes float atomic adds from the kernel: device memory in every layout
+        self.heat_fields = {"t_solid": ManagedArray(small), "q_solid": DeviceField(small, stream),
                             "temp": temp}
         # the temperature planes share the populations' plane stride
-        planes = self.qstride if self.heat_enabled else 1
+        planes = getattr(self, "qstride", n) if self.heat_enabled else 1
         self.g = [zeros((7 * planes,)), zeros((7 * planes,))]
         self._make_heat()
 
@@ -659,11 +740,11 @@ class Solver:
     def q_solid(self):
         """A host copy of q_solid: per sink voxel, the sum of the temperatures of the fluid cells
         facing it, accumulated by the last report step (lbm.py:484-492)."""
-        sync(self.stream)
-        return self.heat_fields["q_solid"].numpy()
+        return self.heat_fields["q_solid"].numpy.copy()
 
     def update_output(self):
-        """lbm.py:606-611: refresh the host-visible output arrays (coarsened when stride > 1)."""
+        """lbm.py:606-611: refresh the host-visible output arrays (coarsened when stride > 1),
+        enqueued on the solver's stream."""
         def source(name):
             return self.heat_fields[name] if name == "temp" else self.fields[name]
         if self.stride > 1:
@@ -676,13 +757,21 @@ class Solver:
                 src = source(name)
                 check(lib().tlbm_memcpy_async(self.output[name].ptr, src.ptr,
                                               4 * self.lat.n, self.stream), "output copy")
+        self._output_stale = False
+
+    def _refresh_output(self):
+        """Bring the output arrays up to date with the last report step and wait for them."""
+        if self._output_stale:
+            self.update_output()
+        sync(self.stream)
 
     def set_launch(self, block_x=0):
-        """Threads per block of this solver's quiet and report kernels (see ``set_launch``)."""
+        """Threads per block of this solver's steps (64, 128, 192 or 256; 0: the default).  The
+        choice belongs to this solver only (it travels in its step parameters)."""
+        if block_x not in (0, 64, 128, 192, 256):
+            raise ValueError(f"block_x must be 0, 64, 128, 192 or 256, got {block_x}")
         self._drop_graphs()
-        for diag in (self.quiet, self.report):
-            set_launch(self.phys, self.geo, diag, self.flow.side, self.storage, block_x,
-                       self.heat)
+        self.block_x = int(block_x)
 
     def set_physics(self, collision, reynolds):
         """lbm.py:613-616: nu = u0 D / Re; a collision change selects another kernel."""
@@ -742,10 +831,12 @@ class Solver:
             self.heat_fields["q_solid"].zero_(self.stream)
         lbm_step(self.lat, self.phys, self.geo, self.flow, diag, self.f[0], self.f[1],
                  self.heat, self.g[0], self.g[1], sums=self.sums if report else None,
-                 stream=self.stream)
+                 stream=self.stream, block_x=getattr(self, "block_x", 0))
         self.f.reverse()
         self.g.reverse()
         self.step_count += 1
+        if report:
+            self._output_stale = True
 
     def run(self, steps, report_every=0):
         """``steps`` steps (even), a report step every ``report_every`` of them (0: none), as
@@ -764,12 +855,16 @@ class Solver:
         if self.heat_enabled and report_every:
             raise ValueError("report steps of the temperature variant clear q_solid with a "
                              "memset, which a graph must not hold: use step()")
-        key = (steps, report_every)
+        # a graph fixes which ping-pong buffer each of its steps reads: key it on the current
+        # order too (an odd number of step() calls between two runs swaps it)
+        key = (steps, report_every, self.f[0].ptr)
         graph = self._graphs.get(key)
         if graph is None:
             graph = self._graphs[key] = self._capture(steps, report_every)
         check(lib().tlbm_graph_launch(graph, self.stream), "graph launch")
         self.step_count += steps
+        if report_every:
+            self._output_stale = True
 
     def _capture(self, steps, report_every):
         L = lib()
@@ -801,8 +896,11 @@ class Solver:
             pass
 
     def monitors(self, interval):
-        """lbm.py:649-661, from the FP64 sums of the last report step."""
-        self.update_output()
+        """lbm.py:649-661, from the FP64 sums of the last report step.  As in the reference,
+        the output grid is coarsened here when stride > 1; at stride 1 the output arrays copy
+        the device fields only when read (``OutputArray``), so monitors itself moves no field."""
+        if self.stride > 1:
+            self.update_output()
         sync(self.stream)
         n = float(self.lat.n)
         s = self.sums.numpy

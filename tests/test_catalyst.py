@@ -106,3 +106,75 @@ def test_channels_of_a_running_solver():
     assert vort["association"] == "element" and vort["values"].size == 23 * 11 * 7
     assert list(_get(tree, "catalyst/channels/derived_status/data/values/message")) == \
         list(b"ok: no steered fields")
+
+
+class _SteerableSolver(_FakeSolver):
+    """Records what steering asks of a Solver (run_lbm.py:34, lbm.py:613-620)."""
+
+    def __init__(self):
+        super().__init__(False)
+        self.step_count = 0
+        self.calls = []
+        self.omega_rot = 0.0
+
+    def set_physics(self, collision, reynolds):
+        self.calls.append(("set_physics", collision, reynolds))
+        self.collision_name, self.reynolds = collision, reynolds
+
+    def set_actuation(self, omega_rot):
+        self.calls.append(("set_actuation", omega_rot))
+        self.omega_rot = omega_rot
+
+
+def _results(parts):
+    """A results tree with one steerable part holding `parts` (key -> array)."""
+    fields = {k: {"values": np.atleast_1d(np.asarray(v, np.float64))} for k, v in parts.items()}
+    return {"catalyst": {"steerable": {"data": {"proxy0": {"fields": fields}}}}}
+
+
+def test_read_steering_takes_both_layouts_and_text():
+    from tlbm.catalyst import read_steering
+    tree = _results({"reynolds": 450.0, "collision": 1.0})
+    tree["catalyst"]["steerable"]["data"]["proxy1"] = {
+        "state": {"fields": {"rotation": np.array([0.75]), "derived": np.array([0])}}}
+    tree["catalyst"]["derived"] = {"data": {"p": {"fields": {"derived": {
+        "values": np.array([ord(c) for c in " w=mag(velocity) "] + [0], np.int32)}}}}}
+    got = read_steering(tree)
+    assert got == {"reynolds": 450.0, "collision": 1.0, "rotation": 0.75, "derived": ""}
+    del tree["catalyst"]["steerable"]
+    assert read_steering(tree) == {"derived": "w=mag(velocity)"}
+    assert read_steering({}) == {}
+
+
+def test_steer_applies_only_what_changed():
+    """run_lbm.py:123-144: unchanged values do nothing, a rotation change only sets the
+    actuation, a Reynolds or collision change calls set_physics with both."""
+    from tlbm.catalyst import CatalystChannels, read_steering
+    s = _SteerableSolver()
+    ch = CatalystChannels(s)
+    assert ch.steer(read_steering(_results({"reynolds": 300.0, "collision": 2.0}))) == {}
+    assert s.calls == []
+    assert ch.steer({"rotation": 0.5}) == {"rotation": 0.5}
+    assert s.calls == [("set_actuation", 0.5 * s.u0)]
+    assert ch.steer({"reynolds": 1000.0, "collision": 0.0, "rotation": 0.5}) == {
+        "reynolds": 1000.0, "collision": 0.0}
+    assert s.calls[-1] == ("set_physics", "bgk", 1000.0)
+    assert ch.steer({"capture_stride": 500.0}) == {"capture_stride": 500.0}
+    assert s.calls[-1] == ("set_physics", "bgk", 1000.0) and len(s.calls) == 2
+    assert ch.steer({"power": 3.0}) == {}             # not a value this solver steers
+    assert ch.steer({"collision": 7.0}) == {"collision": 7.0}
+    assert s.calls[-1] == ("set_physics", "bgk", 1000.0)   # unknown code keeps the model
+
+
+def test_rotation_schedule_fires_each_event_once_in_step_order():
+    from tlbm.catalyst import RotationSchedule
+    s = _SteerableSolver()
+    sched = RotationSchedule("200:-1.5, 100:0.5,,300:2")
+    assert sched.apply(s) == []
+    s.step_count = 150
+    assert sched.apply(s) == [(100, 0.5)] and s.omega_rot == 0.5 * s.u0
+    s.step_count = 400
+    assert sched.apply(s) == [(200, -1.5), (300, 2.0)] and s.omega_rot == 2.0 * s.u0
+    assert sched.apply(s) == []
+    with pytest.raises(ValueError):
+        RotationSchedule("100")

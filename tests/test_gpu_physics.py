@@ -40,7 +40,7 @@ def _run_gpu(cfg, steps, every, storage):
         drag.append(m["drag"])
         lift.append(m["lift"])
     tlbm.sync()
-    u = np.stack([s.fields[k].numpy() for k in ("ux", "uy", "uz")]).astype(np.float64)
+    u = np.stack([s.fields[k].numpy for k in ("ux", "uy", "uz")]).astype(np.float64)
     return np.array(drag), np.array(lift), u
 
 

@@ -155,21 +155,35 @@ def test_ragged_and_degenerate_grids(dims):
     assert _rel(got, ref) < 1e-6
 
 
+def _edge_reps(nx, storage):
+    """The most repeats of an nx x 64 x 4 slab whose two population buffers (19 planes of n
+    cells, packed) still span under 2^32 bytes together with one plane and a row: the largest
+    lattice the kernels address with 32-bit buffer offsets."""
+    size = 4 if storage == "f32" else 2
+    small = nx * 64 * 4
+    reps = ((1 << 32) // size - nx) // (20 * small)
+    assert (20 * small * reps + nx) * size < (1 << 32) <= (20 * small * (reps + 1) + nx) * size
+    return reps
+
+
 @pytest.mark.parametrize("nx", [64, 60])
 @pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_planes_beyond_32_bit_offsets(storage, nx):
+@pytest.mark.parametrize("size", ["beyond", "edge"])
+def test_planes_beyond_32_bit_offsets(storage, nx, size):
     """A lattice of 2^27 cells (64 x 64 x 32768; with nx = 60, 15/16 of that): its population
-    planes end beyond 2^31 elements and beyond 2^32 bytes.  State, old fields and geometry (a
-    rotating z-axis cylinder) repeat every 4 cells along z, so a report step and then a quiet
-    step must give, bit for bit, the steps of the nx x 64 x 4 lattice repeated 8192 times, and
-    FP64 sums 8192 times that lattice's.  The large lattice's quiet step takes the
-    non-temporal-store kernel, the small one's the plain one; with nx = 60 blocks of 256 cells
-    straddle rows and planes."""
+    planes end beyond 2^31 elements and beyond 2^32 bytes ("beyond": per-lane 64-bit addresses);
+    or the largest lattice whose buffers span under 2^32 bytes ("edge": one buffer resource,
+    plane offsets up to 4 GB).  State, old fields and geometry (a rotating z-axis cylinder)
+    repeat every 4 cells along z, so a report step and then a quiet step must give, bit for bit,
+    the steps of the nx x 64 x 4 lattice repeated, and FP64 sums that many times that lattice's.
+    The large lattice's quiet step takes the non-temporal-store kernel, the small one's the
+    plain one; with nx = 60 blocks of 256 cells straddle rows and planes."""
     import tlbm
-    reps = 8192
+    reps = 8192 if size == "beyond" else _edge_reps(nx, storage)
     small = Case((nx, 64, 4), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
     big = Case((nx, 64, 4 * reps), "les", "cylinder", "slip", geometry=(20.3, 30.2, 9.1))
-    assert 19 * big.n > 2 ** 31
+    if size == "beyond":
+        assert 19 * big.n > 2 ** 31
     raw = tlbm.encode_populations(random_populations(small.n, seed=31).astype(np.float32),
                                   storage).reshape(19, 1, small.n)
     old = random_fields(small.n, 32)
@@ -340,7 +354,7 @@ def test_solver_run_graph_replays_match_eager_steps(storage):
         assert mg == me, rep
         assert np.array_equal(g.populations(), e.populations()), rep
         for k in tlbm.FIELDS:
-            assert np.array_equal(g.fields[k].numpy(), e.fields[k].numpy()), (rep, k)
+            assert np.array_equal(g.fields[k].numpy, e.fields[k].numpy), (rep, k)
     assert len(g._graphs) == 1                       # the TRT graph; the LES one was dropped
     with pytest.raises(ValueError):
         g.run(3)
