@@ -7,7 +7,10 @@ quarter in x, rho = 1 with seeded U(-1e-3, 1e-3) velocity noise, and a report st
 residuals, FP64 sums) every 100 steps.  A "step" is one lattice update of the whole grid.
 
 value = whole-grid MLUPS over the timed steps (report steps included) with FP32 populations;
-the FP16/BF16 shifted-storage variants are timed the same way and reported under "variants".
+the FP16 shifted-storage variant is timed the same way and reported under "variants" (BF16 with
+--storage f32,f16,bf16: it misses the 1% Strouhal/drag bar against the FP64 oracle at Re 300).
+Each variant also reports its pattern ceiling: the step's 19-load/19-store access pattern with
+no arithmetic (tlbm_pull_copy) on the same buffers.
 roofline: the quiet FP32 step kernel, 152 algorithmic bytes per cell (19 x 4 B read + 19 x 4 B
 written), over its mean duration from HIP events on the launch stream.
 cpu_baseline: the NumPy FP32 restatement of the same step (the oracle in float32), on a 512x256x4
@@ -24,7 +27,7 @@ replays, under "graph".
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
                        [--no-adjoint] [--no-derived] [--no-heat] [--graph]
-                       [--storage f32,f16,bf16] [--block N]
+                       [--storage f32,f16[,bf16]] [--block N]
                        [--calibrate]
 """
 
@@ -124,6 +127,7 @@ def time_gpu(storage, steps, warmup, block_x=0):
     quiet_n = sum(k for _, _, k in marks)
     m = s.monitors(REPORT_EVERY)
     n = s.lat.n
+    ceiling_ms = time_ceiling(s, stream)
     out = {
         "storage": storage,
         "mlups": n * steps / (total_ms * 1e-3) / 1e6,
@@ -134,9 +138,33 @@ def time_gpu(storage, steps, warmup, block_x=0):
         "monitors": {k: float(v) for k, v in m.items()},
     }
     out["achieved_gbs"] = BYTES_PER_CELL[storage] * n / (out["quiet_kernel_ms"] * 1e-3) / 1e9
+    out["ceiling_ms"] = ceiling_ms
+    out["ceiling_gbs"] = BYTES_PER_CELL[storage] * n / (ceiling_ms * 1e-3) / 1e9
     del s
     L.tlbm_stream_destroy(stream)
     return out
+
+
+def time_ceiling(s, stream, reps=50):
+    """The step's access pattern without its arithmetic (tlbm_pull_copy: 19 pull loads and 19
+    stores per cell, the same mapping, addressing and store policy) on the solver's own two
+    population buffers, after its timed steps: mean ms per launch from HIP events.  The
+    buffers' contents are overwritten."""
+    import tlbm
+    from tlbm._capi import check, lib
+    L = lib()
+    p = tlbm.make_params(s.lat, s.phys, s.geo, s.flow, s.qstride)
+    code = tlbm.STORAGES[s.storage]
+    for _ in range(5):
+        check(L.tlbm_pull_copy(s.f[0].ptr, s.f[1].ptr, ctypes.byref(p), code, stream), "pull copy")
+    ev = Events(L)
+    e0, e1 = ev.new(), ev.new()
+    check(L.tlbm_event_record(e0, stream), "record")
+    for i in range(reps):
+        a, b = (s.f[0], s.f[1]) if i % 2 == 0 else (s.f[1], s.f[0])
+        check(L.tlbm_pull_copy(a.ptr, b.ptr, ctypes.byref(p), code, stream), "pull copy")
+    check(L.tlbm_event_record(e1, stream), "record")
+    return ev.elapsed(e0, e1) / reps
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
@@ -423,7 +451,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--storage", default="f32,f16,bf16")
+    ap.add_argument("--storage", default="f32,f16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0 = default)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched parameter sweep")
@@ -443,9 +471,11 @@ def main():
     results = {}
     for st in storages:
         results[st] = time_gpu(st, args.steps, args.warmup, args.block)
-        print(f"[bench] {st}: {results[st]['mlups']:.0f} MLUPS, quiet kernel "
-              f"{results[st]['quiet_kernel_ms']:.3f} ms = {results[st]['achieved_gbs']:.0f} GB/s",
-              file=sys.stderr, flush=True)
+        r = results[st]
+        print(f"[bench] {st}: {r['mlups']:.0f} MLUPS, quiet kernel {r['quiet_kernel_ms']:.3f} ms "
+              f"= {r['achieved_gbs']:.0f} GB/s; the same access pattern without arithmetic "
+              f"{r['ceiling_ms']:.3f} ms = {r['ceiling_gbs']:.0f} GB/s", file=sys.stderr,
+              flush=True)
     sweep = {}
     if not args.no_sweep:
         for st in ("f32", "f16"):
@@ -509,6 +539,10 @@ def main():
                           "achieved_gbs": round(r["achieved_gbs"], 1),
                           "roofline_frac": round(r["achieved_gbs"] / PEAK_GBS, 4),
                           "traffic": profiled_traffic(st),
+                          "pattern_ceiling": {"ms": round(r["ceiling_ms"], 4),
+                                              "gbs": round(r["ceiling_gbs"], 1),
+                                              "step_frac_of_ceiling": round(
+                                                  r["ceiling_ms"] / r["quiet_kernel_ms"], 4)},
                           "drag": r["monitors"]["drag"], "lift": r["monitors"]["lift"]}
                      for st, r in results.items()},
     }

@@ -152,6 +152,15 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
  * counters and bandwidth of the step are calibrated. */
 int tlbm_copy(void* dst, const void* src, size_t bytes, void* stream);
 
+/* The step's data movement without its arithmetic: f_out[q*qstride + idx] =
+ * f_in[q*qstride + src_q(idx)] for the 19 populations, src_q the pull source of lbm.py:410-415
+ * with x, y and z periodic (collisionless streaming), launched and addressed exactly as the
+ * step's loads and stores (params: nx, ny, nz, qstride; storage picks 4- or 2-byte elements,
+ * copied raw).  No counterpart in the reference: the bandwidth it reaches is the ceiling of the
+ * step's 19-read/19-write access pattern on the device. */
+int tlbm_pull_copy(const void* f_in, void* f_out, const tlbm_params* params, int storage,
+                   void* stream);
+
 /* coarsen (lbm.py:516-522): dst[(k*my + j)*mx + i] = src[((k s)*ny + j s)*nx + i s]. */
 int tlbm_coarsen(const float* src, float* dst, int nx, int ny, int nz, int s, int mx, int my,
                  int mz, void* stream);

@@ -286,6 +286,17 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle, uint32_t elem_bytes
   const uint64_t span = 19ull * static_cast<uint64_t>(p.qstride) + cells +
                         static_cast<uint64_t>(p.nx);
   a.off32 = span * elem_bytes < (1ull << 32) ? 1u : 0u;
+  if (a.off32) {
+    const int64_t plane = static_cast<int64_t>(p.nx) * p.ny;
+    for (int q = 0; q < tlbm::kQ; ++q) {
+      const int64_t shift = tlbm::kCX[q] + tlbm::kCY[q] * static_cast<int64_t>(p.nx) +
+                            tlbm::kCZ[q] * plane;
+      a.plane_off[q] = static_cast<uint32_t>(q * p.qstride * elem_bytes);
+      // plane q's start less its pull shift: >= 0 for q >= 1 since qstride >= cells >= the
+      // shift; q = 0 has no shift
+      a.pull_off[q] = static_cast<uint32_t>((q * p.qstride - shift) * elem_bytes);
+    }
+  }
   a.div_nx = tlbm::make_divisor(static_cast<uint32_t>(p.nx));
   a.div_ny = tlbm::make_divisor(static_cast<uint32_t>(p.ny));
   a.omega = p.omega;
@@ -774,6 +785,33 @@ int tlbm_copy(void* dst, const void* src, size_t bytes, void* stream) {
                      static_cast<uint4*>(dst), n16);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "copy launch");
+  return TLBM_OK;
+}
+
+int tlbm_pull_copy(const void* f_in, void* f_out, const tlbm_params* params, int storage,
+                   void* stream) {
+  if (!f_in || !f_out) return fail(TLBM_ERR_INVALID, "f_in or f_out is NULL");
+  if (f_in == f_out) return fail(TLBM_ERR_INVALID, "f_in and f_out must be distinct buffers");
+  if (storage < TLBM_F32 || storage > TLBM_BF16) return fail(TLBM_ERR_INVALID, "storage %d", storage);
+  if (int rc = check_params(params)) return rc;
+  tlbm::StepArgs a = make_args(*params, TLBM_CYLINDER, storage == TLBM_F32 ? 4u : 2u);
+  a.fin = f_in;
+  a.fout = f_out;
+  const bool nt = stream_stores(19 * static_cast<uint64_t>(params->qstride), storage);
+  const uint64_t n = static_cast<uint64_t>(params->nx) * params->ny * params->nz;
+  const uint64_t nblocks = (n + 255) / 256;
+  if (nblocks * 256 > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
+  const dim3 grid(static_cast<uint32_t>(nblocks)), block(256);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (storage == TLBM_F32) {
+    if (nt) hipLaunchKernelGGL((tlbm::pull_copy_kernel<float, true>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((tlbm::pull_copy_kernel<float, false>), grid, block, 0, st, a);
+  } else {
+    if (nt) hipLaunchKernelGGL((tlbm::pull_copy_kernel<uint16_t, true>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((tlbm::pull_copy_kernel<uint16_t, false>), grid, block, 0, st, a);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "pull copy launch");
   return TLBM_OK;
 }
 

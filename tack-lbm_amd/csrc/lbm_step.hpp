@@ -175,6 +175,9 @@ struct StepArgs {
   uint32_t off32;            // 1: the whole population buffer lies within 2^32 bytes of its
                              // start, so it is addressed as one buffer resource with 32-bit
                              // plane (scalar) and lane (vector) offsets
+  uint32_t pull_off[19];     // off32: byte offset of plane q less the pull shift
+                             // (c_x + c_y nx + c_z nx ny) of population q
+  uint32_t plane_off[19];    // off32: byte offset of plane q
   float omega;               // BGK
   float tau0, smag, magic;   // smag = 25.456 cs^2 (lbm.py:93)
   float o[8];                // obstacle parameters
@@ -514,6 +517,94 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
 }
 
 // ---- XCD-aware block -> tile mapping -----------------------------------------------------------
+// ---- the 19 pull loads of one cell (lbm.py:410-426), raw stored values ------------------------
+// Sources: x and z periodic (lbm.py:412, 415), y by side (lbm.py:342-369): periodic, or the
+// y-mirrored population of the same row at a slip face; ceiling links of the lid read their
+// in-bounds (clamped) row and are overwritten by the caller.
+template <int SIDE, class T>
+__device__ __forceinline__ void pull_raw(const StepArgs& a, const T* __restrict__ fin, int x, int y,
+                                         int z, T (&raw)[kQ]) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const uint32_t snx = static_cast<uint32_t>(nx);
+  const uint32_t plane = snx * static_cast<uint32_t>(ny);
+  const uint32_t idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
+  const int xsrc[3] = {x == nx - 1 ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};
+  const int zsrc[3] = {z == nz - 1 ? 0 : z + 1, z, z == 0 ? nz - 1 : z - 1};
+  int ysrc[3];
+  bool mir_lo = false, mir_hi = false;
+  if constexpr (SIDE == TLBM_PERIODIC_Y) {
+    ysrc[0] = y == ny - 1 ? 0 : y + 1;
+    ysrc[2] = y == 0 ? ny - 1 : y - 1;
+  } else {
+    ysrc[0] = y == ny - 1 ? y : y + 1;
+    ysrc[2] = y == 0 ? y : y - 1;
+    mir_lo = (y == 0);
+    if constexpr (SIDE == TLBM_SLIP_Y) mir_hi = (y == ny - 1);
+  }
+  ysrc[1] = y;
+  // No branch between the 19 loads of a lane, so that all of them are in flight together.
+  // 16-bit storage, interior waves: when no lane of the wave lies on a y or z face, the source
+  // of population q is idx - (cx + cy nx + cz nx ny) for every lane, except the x wrap of one
+  // lane at each row end.  So the 19 loads go through one buffer resource with the plane
+  // offsets in scalar registers (precomputed on the host) and three shared 32-bit lane offsets:
+  // a handful of vector instructions instead of a 64-bit address per load and lane (472 -> 347
+  // VALU instructions per wave of the FP16 LES step, whose VALU was ~100% busy on a box where
+  // it took 0.46 ms).  Waves that touch a y or z face (mirror, lid or wrap per lane) take the
+  // general per-lane path.  FP32 keeps the general path everywhere: it is bandwidth-bound, and
+  // its buffer loads measured 2% slower than per-lane global loads (0.912 against 0.893 ms at
+  // 512x256x256 in one process, same box).
+  const bool inner = y >= 1 && y < ny - 1 && z >= 1 && z < nz - 1;
+  if (sizeof(T) == 2 && a.off32 && __builtin_amdgcn_ballot_w64(!inner) == 0) {
+    constexpr uint32_t B = sizeof(T);
+    const auto rin = plane_rsrc(fin);
+    const uint32_t v0 = idx * B;
+    const uint32_t vp = (idx + (x == 0 ? snx : 0u)) * B;       // c_x = +1: x - 1, wrapped at 0
+    const uint32_t vm = (idx - (x == nx - 1 ? snx : 0u)) * B;  // c_x = -1: x + 1, wrapped
+    static_for<0, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = kCX[q];
+      raw[q] = buffer_load<T>(rin, cx == 1 ? vp : (cx == -1 ? vm : v0), a.pull_off[q]);
+    });
+  } else {
+    static_for<0, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+      int qs = q;
+      if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = mir_lo ? miry(q) : q;
+      if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = mir_hi ? miry(q) : q;
+      const T* __restrict__ row = fin + static_cast<uint64_t>(qs) * a.ns +
+                                  static_cast<uint32_t>(zsrc[1 + cz]) * plane +
+                                  static_cast<uint32_t>(ysrc[1 + cy]) * snx;
+      raw[q] = row[xsrc[1 + cx]];
+    });
+  }
+}
+
+// ---- the 19 stores of one cell -------------------------------------------------------------------
+// NT: non-temporal stores, for lattices whose two population buffers are far larger than the
+// Infinity Cache (measured on MI355X: 1.6% faster at FP32, 3.4% at FP16, 512x256x256); below
+// that size the next step's loads hit the cache lines plain stores leave there.
+template <bool NT, class T>
+__device__ __forceinline__ void store_raw(const StepArgs& a, T* __restrict__ fout, uint32_t idx,
+                                          const T (&raw)[kQ]) {
+  if (sizeof(T) == 2 && a.off32) {
+    // one lane offset for all 19 planes, the plane offsets in scalar registers
+    constexpr uint32_t B = sizeof(T);
+    const auto rout = plane_rsrc(fout);
+    static_for<0, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      buffer_store<T, NT>(raw[q], rout, idx * B, a.plane_off[q]);
+    });
+  } else {
+    static_for<0, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      T* const dst = (fout + static_cast<uint64_t>(q) * a.ns) + idx;
+      if constexpr (NT) __builtin_nontemporal_store(raw[q], dst);
+      else *dst = raw[q];
+    });
+  }
+}
+
 // ---- the fused step of one cell ------------------------------------------------------------------
 // Pull, bounce-back, collision, sponge and store for cell (x, y, z) of the lattice described by
 // `a`, reading `fin` and writing `fout` (plane stride a.ns); x, y and z are per lane.  REPORT: writes the 13 fields at `fidx` (when F.ux is not null) and returns the 14 FP64
@@ -646,41 +737,11 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   ysrc[1] = y;
 
   // ---- pull streaming (lbm.py:410-426) ----
-  // No branch between the 19 loads, so that all of them are in flight together: ceiling links
-  // of the lid read their in-bounds (clamped) row and are overwritten below.
   float f[kQ];
-  // Interior waves: when no lane of the wave lies on a y or z face, the source of population q
-  // is idx - (cx + cy nx + cz nx ny) for every lane, except the x wrap of one lane at each row
-  // end.  So each plane gets one wave-uniform base pointer (scalar registers) and the 19 loads
-  // share three 32-bit lane offsets: a handful of vector instructions instead of a 64-bit
-  // address per load and lane, which made the 16-bit kernels instruction-bound.  Waves that
-  // touch a y or z face (mirror, lid or wrap per lane) take the general per-lane path.
-  const bool inner = y >= 1 && y < ny - 1 && z >= 1 && z < nz - 1;
-  if (a.off32 && __builtin_amdgcn_ballot_w64(!inner) == 0) {
-    constexpr uint32_t B = sizeof(T);
-    const auto rin = plane_rsrc(fin);
-    const uint32_t v0 = idx * B;
-    const uint32_t vp = (idx + (x == 0 ? snx : 0u)) * B;       // c_x = +1: x - 1, wrapped at 0
-    const uint32_t vm = (idx - (x == nx - 1 ? snx : 0u)) * B;  // c_x = -1: x + 1, wrapped
-    static_for<0, kQ>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-      const int64_t shift = cx + cy * static_cast<int64_t>(nx) + cz * static_cast<int64_t>(plane);
-      const uint32_t soff = static_cast<uint32_t>((static_cast<int64_t>(q) * a.ns - shift) * B);
-      f[q] = St::value(buffer_load<T>(rin, cx == 1 ? vp : (cx == -1 ? vm : v0), soff));
-    });
-  } else {
-    static_for<0, kQ>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-      int qs = q;
-      if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = mir_lo ? miry(q) : q;
-      if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = mir_hi ? miry(q) : q;
-      const T* __restrict__ row = fin + static_cast<uint64_t>(qs) * a.ns +
-                                  static_cast<uint32_t>(zsrc[1 + cz]) * plane +
-                                  static_cast<uint32_t>(ysrc[1 + cy]) * snx;
-      f[q] = St::value(row[xsrc[1 + cx]]);
-    });
+  {
+    T raw[kQ];
+    pull_raw<SIDE>(a, fin, x, y, z, raw);
+    static_for<0, kQ>([&](auto qc) { f[decltype(qc)::value] = St::value(raw[decltype(qc)::value]); });
   }
 
   // ---- moving lid: f_q = f_opp(x) + 6 w_q c_q . u_lid on ceiling links ----
@@ -754,26 +815,9 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   CollideExtra ex;
   collide_cell<COLL, kShift, HEAT>(a, f, static_cast<float>(x), here, rho, ux, uy, uz, &ex);
   if (active) {
-    // NT: non-temporal stores, for lattices whose two population buffers are far larger than
-    // the Infinity Cache (measured on MI355X: 1.6% faster at FP32, 3.4% at FP16, 512x256x256);
-    // below that size the next step's loads hit the cache lines plain stores leave there.
-    if (a.off32) {
-      // one lane offset for all 19 planes, the plane offsets in scalar registers
-      constexpr uint32_t B = sizeof(T);
-      const auto rout = plane_rsrc(fout);
-      static_for<0, kQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        buffer_store<T, NT>(St::encode(f[q]), rout, idx * B,
-                            static_cast<uint32_t>(static_cast<uint64_t>(q) * a.ns * B));
-      });
-    } else {
-      static_for<0, kQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        T* const dst = (fout + static_cast<uint64_t>(q) * a.ns) + idx;
-        if constexpr (NT) __builtin_nontemporal_store(St::encode(f[q]), dst);
-        else *dst = St::encode(f[q]);
-      });
-    }
+    T raw[kQ];
+    static_for<0, kQ>([&](auto qc) { raw[decltype(qc)::value] = St::encode(f[decltype(qc)::value]); });
+    store_raw<NT>(a, fout, idx, raw);
   }
   if constexpr (HEAT) heat_cell<OBS, REPORT, NT>(a, x, y, z, idx, here, ux, uy, uz, ex, active);
 
@@ -858,6 +902,23 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   if constexpr (REPORT) {
     if (a.partials) block_sums(acc, a.partials, a.nblocks, blockIdx.x);
   }
+}
+
+// ---- the step's data movement alone --------------------------------------------------------------
+// f_out[q][idx] = f_in[q][source of q] for the 19 populations, the sources of a periodic lattice,
+// with the step's mapping, index split, loads and stores and no arithmetic: the bandwidth this
+// access pattern reaches is the ceiling of the fused step.  T: the stored element (4 or 2 bytes).
+template <class T, bool NT>
+__global__ void __launch_bounds__(256) pull_copy_kernel(const StepArgs a) {
+  const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * static_cast<uint32_t>(a.nz);
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  uint32_t x, y, z;
+  cell_coords(idx, a.nx, a.ny, a.div_nx, a.div_ny, x, y, z);
+  T raw[kQ];
+  pull_raw<TLBM_PERIODIC_Y>(a, static_cast<const T*>(a.fin), static_cast<int>(x),
+                            static_cast<int>(y), static_cast<int>(z), raw);
+  store_raw<NT>(a, static_cast<T*>(a.fout), idx, raw);
 }
 
 // ---- batched lattices: many independent domains in one persistent launch -------------------------

@@ -7,8 +7,9 @@ HIP for gfx950, C ABI in include/tlbm.h).
 
 from . import _capi  # noqa: F401
 from . import adjoint, autotune, batch  # noqa: F401
-from .lbm import (BGK, COLLISIONS, CX, CY, CZ, FIELDS, TRT, W, Box, Cylinder, D3Q19,  # noqa: F401
-                  DeviceArray, Flow, Heat, HeatSink, HostBuffer, LidY, ManagedArray, NoHeat, PeriodicY, Quiet,
-                  Report, SlipY, SmagorinskyLES, SolidField, Solver, Sphere, array, coarsen,
-                  set_launch,
-                  decode_populations, empty, encode_populations, lbm_init, lbm_step, sync, zeros)
+from .lbm import (BGK, COLLISIONS, CX, CY, CZ, FIELDS, STORAGES, TRT, W, Box,  # noqa: F401
+                  Cylinder, D3Q19, DeviceArray, DeviceField, Flow, Heat, HeatSink, HostBuffer,
+                  LidY, ManagedArray, NoHeat, OutputArray, PeriodicY, Quiet, Report, SlipY,
+                  SmagorinskyLES, SolidField, Solver, Sphere, array, coarsen, decode_populations,
+                  empty, encode_populations, kernel_handle, lbm_init, lbm_step, make_params,
+                  set_launch, sync, zeros)

@@ -81,6 +81,7 @@ SIGNATURES = {
     "tlbm_step": (_c.c_int, [_c.c_int, _vp, _vp, _c.POINTER(Params), _c.POINTER(Fields),
                              _vp, _vp]),
     "tlbm_copy": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
+    "tlbm_pull_copy": (_c.c_int, [_vp, _vp, _c.POINTER(Params), _c.c_int, _vp]),
     "tlbm_batch_create": (_c.c_int, [_c.c_int, _c.POINTER(Params), _c.c_int, _c.POINTER(_vp)]),
     "tlbm_batch_destroy": (_c.c_int, [_vp]),
     "tlbm_batch_layout": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_uint64),

@@ -71,5 +71,6 @@ def test_mid_run_switch_matches_the_oracle_switched_at_the_same_step(target):
     moved = float(np.abs(plain.f - want).max() / np.abs(want).max())
     print(f"steer {name} Re {re}: populations {err:.2e}, drag {drag_err:.2e}, "
           f"unswitched differs by {moved:.2e}")
-    assert err < 2e-6 and drag_err < 1e-4
-    assert moved > 100 * 2e-6
+    # FP32 against FP64 over 60 steps (the unsteered C1 runs hold 3.4e-5 u0 over 1000 steps)
+    assert err < 2e-5 and drag_err < 1e-4
+    assert moved > 100 * 2e-5
