@@ -147,6 +147,19 @@ int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, fl
 int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* params,
               const tlbm_fields* fields, double* sums, void* stream);
 
+/* tlbm_step without any device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as tlbm_step; a report step's per-block FP64
+ * partials go to a fixed array the library holds on each device it runs on (set up at the
+ * first call on that device) instead of a stream workspace, so report steps of lattices of
+ * more than TLBM_NOALLOC_MAX_BLOCKS blocks (at the default 256 threads per block:
+ * 33,554,432 cells; 14.7 MB per device) are refused with TLBM_ERR_INVALID.  Every call on one
+ * device shares that array: report steps of overlapping calls (other streams, other threads)
+ * race, so serialise them. */
+#define TLBM_NOALLOC_MAX_BLOCKS 131072
+int tlbm_step_noalloc(int handle, const void* f_in, void* f_out, const tlbm_params* params,
+                      const tlbm_fields* fields, double* sums, void* stream);
+
 /* A plain device-to-device copy kernel (16 bytes per lane; pointers and size 16-byte aligned).
  * No counterpart in the reference: it is the known-size streaming kernel against which HBM
  * counters and bandwidth of the step are calibrated. */

@@ -210,6 +210,28 @@ void release_workspaces(hipStream_t stream) {
   g_ws_captured.erase(stream);
 }
 
+// ---- partials of tlbm_step_noalloc: a fixed device array, no allocation ---------------------
+// Each device holds its own copy; host code must use the address hipGetSymbolAddress gives on
+// that device (the array's name is only a host-side handle), looked up once per device.
+static __device__ __align__(256) double scratch[TLBM_NSUMS * TLBM_NOALLOC_MAX_BLOCKS];
+std::mutex g_scratch_mutex;
+std::map<int, double*> g_scratch_ptr;
+
+int scratch_on_current_device(double** out) {
+  int dev = 0;
+  TLBM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void* p = nullptr;
+    TLBM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(TLBM_ERR_HIP, "hipGetSymbolAddress gave NULL for the partials array");
+    it = g_scratch_ptr.emplace(dev, static_cast<double*>(p)).first;
+  }
+  *out = it->second;
+  return TLBM_OK;
+}
+
 // Sums partials[k][0..nblocks) in a fixed order (block k sums row k): deterministic FP64 totals.
 __global__ void __launch_bounds__(1024) reduce_partials(const double* __restrict__ partials,
                                                         uint32_t nblocks, double* __restrict__ out) {
@@ -675,9 +697,12 @@ int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, fl
 namespace {
 
 // tlbm_step and tlbm_step_heat: `heat` is NULL for the flow step alone.
+// scratch: a caller-provided partials buffer of scratch_blocks blocks (the no-allocation entry
+// point), or NULL for the stream's workspace.
 int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const float* g_in,
               float* g_out, const tlbm_params* params, const tlbm_heat* heat,
-              const tlbm_fields* fields, double* sums, void* stream) {
+              const tlbm_fields* fields, double* sums, void* stream, double* scratch = nullptr,
+              uint32_t scratch_blocks = 0) {
   if (!f_in || !f_out) return fail(TLBM_ERR_INVALID, "f_in or f_out is NULL");
   if (f_in == f_out) return fail(TLBM_ERR_INVALID, "f_in and f_out must be distinct buffers");
   if (int rc = check_params(params, d.obstacle)) return rc;
@@ -729,8 +754,15 @@ int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const
       a.fld = *fields;
     }
     if (sums) {
-      double* ws = nullptr;
-      if (int rc = workspace(sizeof(double) * TLBM_NSUMS * nblocks, st, &ws)) return rc;
+      double* ws = scratch;
+      if (scratch) {
+        if (nblocks > scratch_blocks)
+          return fail(TLBM_ERR_INVALID,
+                      "report step of %u blocks exceeds the %u blocks of the no-allocation "
+                      "partials array", nblocks, scratch_blocks);
+      } else if (int rc = workspace(sizeof(double) * TLBM_NSUMS * nblocks, st, &ws)) {
+        return rc;
+      }
       a.partials = ws;
       a.nblocks = nblocks;
     }
@@ -759,6 +791,20 @@ int tlbm_step(int handle, const void* f_in, void* f_out, const tlbm_params* para
   if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
   if (d.heat) return fail(TLBM_ERR_INVALID, "handle %d has heat: step it with tlbm_step_heat", handle);
   return step_impl(handle, d, f_in, f_out, nullptr, nullptr, params, nullptr, fields, sums, stream);
+}
+
+int tlbm_step_noalloc(int handle, const void* f_in, void* f_out, const tlbm_params* params,
+                      const tlbm_fields* fields, double* sums, void* stream) {
+  std::call_once(g_once, build_tables);
+  Decoded d;
+  if (!decode(handle, d)) return fail(TLBM_ERR_INVALID, "bad kernel handle %d", handle);
+  if (d.heat) return fail(TLBM_ERR_INVALID, "handle %d has heat: step it with tlbm_step_heat", handle);
+  double* partials = nullptr;
+  if (int rc = scratch_on_current_device(&partials)) return rc;
+  // every block of a report step writes all TLBM_NSUMS of its own partials before the
+  // reduction reads them, so the array needs no clearing between calls
+  return step_impl(handle, d, f_in, f_out, nullptr, nullptr, params, nullptr, fields, sums, stream,
+                   partials, TLBM_NOALLOC_MAX_BLOCKS);
 }
 
 int tlbm_step_heat(int handle, const void* f_in, void* f_out, const float* g_in, float* g_out,

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libtlbm.so")
 
 TLBM_OK = 0
 TLBM_NSUMS = 14
+TLBM_NOALLOC_MAX_BLOCKS = 131072
 
 BGK, TRT, LES = 0, 1, 2
 CYLINDER, SPHERE, BOX, FIELD, HEATSINK = 0, 1, 2, 3, 4
@@ -80,6 +81,8 @@ SIGNATURES = {
                              _c.c_uint64, _vp]),
     "tlbm_step": (_c.c_int, [_c.c_int, _vp, _vp, _c.POINTER(Params), _c.POINTER(Fields),
                              _vp, _vp]),
+    "tlbm_step_noalloc": (_c.c_int, [_c.c_int, _vp, _vp, _c.POINTER(Params), _c.POINTER(Fields),
+                                     _vp, _vp]),
     "tlbm_copy": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
     "tlbm_pull_copy": (_c.c_int, [_vp, _vp, _c.POINTER(Params), _c.c_int, _vp]),
     "tlbm_batch_create": (_c.c_int, [_c.c_int, _c.POINTER(Params), _c.c_int, _c.POINTER(_vp)]),
