@@ -22,11 +22,12 @@ adjoint: the gradient workload (BGK 128x64x64, 2000 steps checkpointed every 100
 backward pass times and the adjoint kernel against the forward step kernel, under "adjoint".
 derived: vorticity magnitude and Q-criterion of the 512x256x256 output grid, under "derived".
 heat: the temperature variant's step against the flow step (heat-sink lattice), under "heat".
+driver_loop: the reference's loop shape (99 quiet steps, a report step, monitors), wall clock.
 graph (with --graph): a small lattice stepped one step() at a time and by Solver.run hipGraph
 replays, under "graph".
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
-                       [--no-adjoint] [--no-derived] [--no-heat] [--graph]
+                       [--no-adjoint] [--no-derived] [--no-heat] [--no-loop] [--graph]
                        [--storage f32,f16[,bf16]] [--block N]
                        [--calibrate]
 """
@@ -143,6 +144,40 @@ def time_gpu(storage, steps, warmup, block_x=0):
     del s
     L.tlbm_stream_destroy(stream)
     return out
+
+
+def time_loop(storage="f32", intervals=3):
+    """The reference's driver loop shape (run_lbm.py:207-217, collision_check.py:14-17) on the
+    headline workload: per interval 99 quiet steps, one report step and ``monitors(100)``, wall
+    clock around `intervals` intervals; then the same with an in-situ reader taking the five
+    output arrays (OutputArray.numpy: the report fields copied into managed memory)."""
+    import tlbm
+    from tlbm._capi import check, lib
+    L = lib()
+    stream = ctypes.c_void_p()
+    check(L.tlbm_stream_create(ctypes.byref(stream)), "stream create")
+    s = make_solver(storage, stream.value)
+    out = {}
+    for reader in (False, True):
+        for _ in range(REPORT_EVERY):                       # warm-up interval
+            s.step(report=(s.step_count + 1) % REPORT_EVERY == 0)
+        s.monitors(REPORT_EVERY)
+        t0 = time.perf_counter()
+        for _ in range(intervals):
+            for _ in range(REPORT_EVERY - 1):
+                s.step()
+            s.step(report=True)
+            s.monitors(REPORT_EVERY)
+            if reader:
+                for k in s.OUTPUT:
+                    s.output[k].numpy
+        ms = (time.perf_counter() - t0) * 1e3 / (intervals * REPORT_EVERY)
+        out["with_output_read" if reader else "monitors_only"] = ms
+    del s
+    L.tlbm_stream_destroy(stream)
+    return {"workload": "headline, 99 quiet steps + 1 report step + monitors(100) per interval",
+            "storage": storage, "ms_per_step": round(out["monitors_only"], 4),
+            "ms_per_step_with_output_read": round(out["with_output_read"], 4)}
 
 
 def time_ceiling(s, stream, reps=50):
@@ -458,6 +493,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--no-derived", action="store_true", help="skip the derived-field timing")
     ap.add_argument("--no-heat", action="store_true", help="skip the temperature-variant timing")
+    ap.add_argument("--no-loop", action="store_true",
+                    help="skip the driver-loop timing (report step + monitors every 100)")
     ap.add_argument("--graph", action="store_true",
                     help="add the small-lattice step-by-step vs hipGraph timing")
     ap.add_argument("--calibrate", action="store_true",
@@ -501,6 +538,12 @@ def main():
         print(f"[bench] heat: step {heat['heat_step_ms']:.3f} ms vs flow {heat['flow_step_ms']:.3f} "
               f"ms ({heat['heat_over_flow']:.2f}x, {heat['achieved_gbs']:.0f} GB/s)",
               file=sys.stderr, flush=True)
+    loop = None
+    if not args.no_loop:
+        loop = time_loop()
+        print(f"[bench] driver loop (99 quiet + report + monitors): {loop['ms_per_step']:.4f} ms "
+              f"per step; with the output grid read each interval "
+              f"{loop['ms_per_step_with_output_read']:.4f} ms", file=sys.stderr, flush=True)
     graph = None
     if args.graph:
         graph = time_graph()
@@ -554,6 +597,8 @@ def main():
                            for k, v in adjoint.items()}
     if heat:
         line["heat"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in heat.items()}
+    if loop:
+        line["driver_loop"] = loop
     if graph:
         line["graph"] = {k: (round(v, 4) if isinstance(v, float) else v)
                          for k, v in graph.items()}
