@@ -13,9 +13,11 @@ Differences, by design:
     13 managed fields on the host (lbm.py:654);
   * the 13 report fields live in HBM by default (``field_memory="device"``): managed memory
     on an MI355X without XNACK is host-resident, and writing the fields there each report step
-    would cost ~30x the step itself.  ``Solver.output`` stays a set of managed arrays that
-    in-situ readers use zero-copy; ``update_output()`` (called by ``monitors``) refreshes them.
-    ``field_memory="managed"`` keeps the reference's layout (the kernel writes managed memory);
+    would cost ~30x the step itself.  Host code reads them as it reads managed arrays
+    (``DeviceField.numpy`` is a synchronised host copy).  ``Solver.output`` stays a set of
+    managed arrays that in-situ readers use zero-copy (``OutputArray``): reading one after a
+    report step refreshes it from the device field.  ``field_memory="managed"`` keeps the
+    reference's layout (the kernel writes managed memory);
   * population storage can be FP16 or BF16 deviations f_q - w_q (``storage="f16"/"bf16"``);
   * a Solver of 2^20 cells or more pads the population planes (``default_qstride``): plane q of
     ``Solver.f`` starts at ``q * qstride``, not ``q * n`` (the kernels take the stride);
