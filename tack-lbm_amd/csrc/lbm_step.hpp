@@ -790,12 +790,15 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
         const float b = St::value(own[q]);
         float t = 0.0f;
         if constexpr (Obs::kMoving) {
-          // wall velocity at the link midpoint (lbm.py:418-425)
-          float uwx, uwy, uwz;
-          Obs::wall_vel(a, static_cast<float>(x) - 0.5f * cx, static_cast<float>(y) - 0.5f * cy,
-                        static_cast<float>(z) - 0.5f * cz, uwx, uwy, uwz);
-          t = 6.0f * kW[q] *
-              (cx * (omega_rot * uwx) + cy * (omega_rot * uwy) + cz * (omega_rot * uwz));
+          // wall velocity at the link midpoint (lbm.py:418-425); a surface at rest (the
+          // launch-wide omega_rot = 0) gives t = 0 exactly, so its divisions are skipped
+          if (omega_rot != 0.0f) {
+            float uwx, uwy, uwz;
+            Obs::wall_vel(a, static_cast<float>(x) - 0.5f * cx, static_cast<float>(y) - 0.5f * cy,
+                          static_cast<float>(z) - 0.5f * cz, uwx, uwy, uwz);
+            t = 6.0f * kW[q] *
+                (cx * (omega_rot * uwx) + cy * (omega_rot * uwy) + cz * (omega_rot * uwz));
+          }
         }
         // lbm.py:428-429; with a 0/1 obstacle wall = 1 here, so f = b + t exactly
         if constexpr (OBS == TLBM_FIELD) f[q] = f[q] + wall * (b - f[q] + t);
