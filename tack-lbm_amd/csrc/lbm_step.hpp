@@ -774,10 +774,19 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
     // issued together, before any is needed, so a wave near the wall waits for memory once
     // rather than once per wall link.
     T own[kQ];
-    static_for<1, kQ>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      own[q] = (fin + static_cast<uint64_t>(opp(q)) * a.ns)[idx];
-    });
+    if (sizeof(T) == 2 && a.off32) {
+      // one lane offset, the planes' offsets in scalar registers (as the 16-bit pulls)
+      const auto rin = plane_rsrc(fin);
+      static_for<1, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        own[q] = buffer_load<T>(rin, idx * static_cast<uint32_t>(sizeof(T)), a.plane_off[opp(q)]);
+      });
+    } else {
+      static_for<1, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        own[q] = (fin + static_cast<uint64_t>(opp(q)) * a.ns)[idx];
+      });
+    }
     static_for<1, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
