@@ -269,7 +269,8 @@ int tlbm_batch_step(tlbm_batch* batch, const void* f_in, void* f_out, int report
                     const tlbm_fields* fields, double* sums, void* stream);
 /* Capture `steps` (even) consecutive batch steps, f_a -> f_b -> f_a ..., with a report step
  * every `report_every` steps (0: none), on `stream` (not the null stream) into a hipGraph;
- * launching the graph runs them all and leaves the state in f_a. */
+ * launching the graph runs them all and leaves the state in f_a.  The graph uses the batch's
+ * own tables and partials: destroy it before tlbm_batch_set_launch or tlbm_batch_destroy. */
 int tlbm_batch_graph(tlbm_batch* batch, void* f_a, void* f_b, int steps, int report_every,
                      const tlbm_fields* fields, double* sums, void* stream, tlbm_graph** out);
 int tlbm_graph_launch(tlbm_graph* graph, void* stream);
@@ -280,7 +281,11 @@ int tlbm_graph_destroy(tlbm_graph* graph);
  * be captured; nothing that allocates: the first report step with sums on a stream (or on a
  * larger lattice than before) sizes its reduction workspace and must run eagerly first, and
  * fails with TLBM_ERR_INVALID under capture; tlbm_memset is refused.  When a call fails under
- * capture, end the capture and destroy what tlbm_capture_end returns. */
+ * capture, end the capture and destroy what tlbm_capture_end returns.  A graph keeps the
+ * reduction workspaces its report steps use: they stay allocated, even when the stream needs a
+ * larger one or is destroyed, until the last graph using them is destroyed.  A graph replays
+ * the buffers it was captured with (a caller alternating two buffers captures one graph per
+ * order). */
 int tlbm_capture_begin(void* stream);
 int tlbm_capture_end(void* stream, tlbm_graph** out);
 /* Compute units and architecture name of the current device (sizing persistent grids, keying
