@@ -96,7 +96,18 @@ def _first(value):
     return float(np.atleast_1d(np.asarray(value, dtype=np.float64)).reshape(-1)[0])
 
 
+def _is_node(obj):
+    """A conduit.Node (what catalyst.results fills), told apart by its path API."""
+    return hasattr(obj, "has_path") and hasattr(obj, "number_of_children")
+
+
 def _lookup(tree, path):
+    """The value at `path` of a nested dict or a conduit.Node; None where it is missing."""
+    if _is_node(tree):
+        if not tree.has_path(path):
+            return None
+        node = tree[path]
+        return node if node.number_of_children() else node.value()
     node = tree
     for part in path.split("/"):
         if not isinstance(node, dict) or part not in node:
@@ -105,25 +116,27 @@ def _lookup(tree, path):
     return node
 
 
+def _parts(data):
+    """The children of a mesh's data node (a dict's values or a conduit.Node's children)."""
+    if _is_node(data):
+        return [data.child(i) for i in range(data.number_of_children())]
+    return list(data.values()) if isinstance(data, dict) else []
+
+
 def read_steering(results):
-    """steering.py:124-147 over a results tree (nested dicts keyed by path component, as
-    ``CatalystChannels.execute`` builds them): the derived-field request text and the steered
-    scalars of every part under ``catalyst/derived/data`` and ``catalyst/steerable/data``, each
-    from ``state/fields/<key>`` or ``fields/<key>/values``.  Missing values are left out."""
+    """steering.py:124-147 over a results tree: a conduit.Node (what ``catalyst.results``
+    fills) or nested dicts keyed by path component (as ``CatalystChannels.execute`` builds
+    them).  Returns the derived-field request text and the steered scalars of every part under
+    ``catalyst/derived/data`` and ``catalyst/steerable/data``, each from
+    ``state/fields/<key>`` or ``fields/<key>/values``; missing values are left out."""
     values = {}
     for mesh in ("catalyst/derived/data", "catalyst/steerable/data"):
-        data = _lookup(results, mesh)
-        if not isinstance(data, dict):
-            continue
-        for part in data.values():
+        for part in _parts(_lookup(results, mesh)):
             for path in ("state/fields/derived", "fields/derived/values"):
                 v = _lookup(part, path)
                 if v is not None:
                     values["derived"] = _text(v)
-    data = _lookup(results, "catalyst/steerable/data")
-    if not isinstance(data, dict):
-        return values
-    for part in data.values():
+    for part in _parts(_lookup(results, "catalyst/steerable/data")):
         for key in STEERABLE:
             for path in (f"state/fields/{key}", f"fields/{key}/values"):
                 v = _lookup(part, path)

@@ -178,3 +178,55 @@ def test_rotation_schedule_fires_each_event_once_in_step_order():
     assert sched.apply(s) == []
     with pytest.raises(ValueError):
         RotationSchedule("100")
+
+
+class _Node:
+    """The part of conduit.Node's API that read_steering uses, over a nested dict."""
+
+    def __init__(self, tree):
+        self.tree = tree
+
+    def _get(self, path):
+        node = self.tree
+        for part in path.split("/"):
+            node = node[part]
+        return node
+
+    def has_path(self, path):
+        try:
+            self._get(path)
+            return True
+        except (KeyError, TypeError):
+            return False
+
+    def __getitem__(self, path):
+        v = self._get(path)
+        return _Node(v) if isinstance(v, dict) else _Leaf(v)
+
+    def number_of_children(self):
+        return len(self.tree)
+
+    def child(self, i):
+        return _Node(list(self.tree.values())[i])
+
+
+class _Leaf:
+    def __init__(self, v):
+        self.v = v
+
+    def number_of_children(self):
+        return 0
+
+    def value(self):
+        return self.v
+
+
+def test_read_steering_takes_a_conduit_node():
+    """catalyst.results fills a conduit.Node: the same values as from the dict tree."""
+    from tlbm.catalyst import read_steering
+    tree = _results({"reynolds": 450.0, "collision": 1.0, "rotation": -0.25})
+    tree["catalyst"]["derived"] = {"data": {"p": {"state": {"fields": {
+        "derived": np.array([ord(c) for c in "v=mag(velocity)"], np.int32)}}}}}
+    assert read_steering(_Node(tree)) == read_steering(tree) == {
+        "reynolds": 450.0, "collision": 1.0, "rotation": -0.25, "derived": "v=mag(velocity)"}
+    assert read_steering(_Node({})) == {}
