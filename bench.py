@@ -258,6 +258,7 @@ def time_sweep(storage, steps=5000, graph_steps=100, seq_steps=20):
     b = BatchSolver(domains, collision="trt", storage=storage, noise_amp=1e-3,
                     stream=stream.value)
     block, grid, tune_ms = autotune.tune_batch(b)
+    group = b.group
     b.run(graph_steps, report_every=graph_steps)  # capture + warm-up
     ev = Events(L)
     e0, e1 = ev.new(), ev.new()
@@ -290,7 +291,8 @@ def time_sweep(storage, steps=5000, graph_steps=100, seq_steps=20):
     return {"domains": len(domains), "cells": cells, "steps": steps,
             "mlups": cells * steps / (ms * 1e-3) / 1e6, "ms_per_step": ms / steps,
             "achieved_gbs": bpc * cells * steps / (ms * 1e-3) / 1e9,
-            "launch": {"block": block, "persistent_grid": grid, "tuned_ms": round(tune_ms, 4)},
+            "launch": {"block": block, "persistent_grid": grid, "xcd_chunk_group": group,
+                       "tuned_ms": round(tune_ms, 4)},
             "one_launch_per_domain_mlups": cells * seq_steps / (seq_ms * 1e-3) / 1e6,
             "drag_range": [min(m["drag"] for m in mons), max(m["drag"] for m in mons)]}
 

@@ -258,6 +258,12 @@ int tlbm_batch_layout(const tlbm_batch* batch, int domain, uint64_t* elem_offset
 /* Threads per block (64, 128, 192, 256; 0 = 256) and persistent grid size in blocks (0 = one block
  * per chunk).  Changing block_x synchronises the device and rebuilds the chunk tables. */
 int tlbm_batch_set_launch(tlbm_batch* batch, int block_x, int grid_blocks);
+/* Chunk order of the batch's launches: group = 1 (the default) takes the chunks in plain
+ * order, consecutive chunks on consecutive XCDs; group = g > 1 gives each XCD runs of g
+ * consecutive chunks, so the cache lines neighbouring chunks share are fetched into one L2
+ * (a persistent grid is then rounded down to a multiple of 8 blocks).  Results do not depend
+ * on the order.  No counterpart in the reference (tuned by tlbm.autotune). */
+int tlbm_batch_set_order(tlbm_batch* batch, int group);
 /* lbm_init of every domain (see tlbm_init). */
 int tlbm_batch_init(tlbm_batch* batch, void* f, float seed_amp, float noise_amp,
                     uint64_t noise_seed, void* stream);

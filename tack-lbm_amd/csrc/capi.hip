@@ -419,6 +419,7 @@ struct tlbm_batch {
   bool nt = false;  // quiet steps use non-temporal population stores (large batches)
   int block = 256;
   int grid = 0;  // persistent grid (blocks); 0 = one block per chunk
+  uint32_t group = 1;  // consecutive chunks per XCD run (batch_chunk)
   tlbm::BatchDomain* d_dom = nullptr;
   uint16_t* d_chunk_dom = nullptr;
   uint32_t* d_chunk_off = nullptr;
@@ -946,6 +947,13 @@ int tlbm_batch_set_launch(tlbm_batch* b, int block_x, int grid_blocks) {
   return TLBM_OK;
 }
 
+int tlbm_batch_set_order(tlbm_batch* b, int group) {
+  if (!b) return fail(TLBM_ERR_INVALID, "batch is NULL");
+  if (group < 1 || group > 64) return fail(TLBM_ERR_INVALID, "group must be 1..64 (got %d)", group);
+  b->group = static_cast<uint32_t>(group);
+  return TLBM_OK;
+}
+
 int tlbm_batch_init(tlbm_batch* b, void* f, float seed_amp, float noise_amp, uint64_t noise_seed,
                     void* stream) {
   if (!b || !f) return fail(TLBM_ERR_INVALID, "batch or f is NULL");
@@ -974,6 +982,9 @@ int tlbm_batch_step(tlbm_batch* b, const void* f_in, void* f_out, int report,
   a.chunk_dom = b->d_chunk_dom;
   a.ndom = static_cast<int32_t>(b->params.size());
   a.nchunks = b->nchunks;
+  a.group = b->group;
+  const uint64_t run = 8ull * b->group;
+  a.nslots = b->group > 1 ? static_cast<uint32_t>((b->nchunks + run - 1) / run * run) : b->nchunks;
   if (report) {
     if (fields) {
       const float* const* ptrs = reinterpret_cast<const float* const*>(fields);
@@ -983,8 +994,10 @@ int tlbm_batch_step(tlbm_batch* b, const void* f_in, void* f_out, int report,
     }
     a.partials = sums ? b->d_partials : nullptr;
   }
-  // one block per chunk, or a persistent grid that strides over the chunks
-  const uint32_t grid = b->grid > 0 ? std::min<uint32_t>(b->grid, b->nchunks) : b->nchunks;
+  // one block per chunk slot, or a persistent grid that strides over the slots (a multiple of
+  // 8 blocks when chunks run in groups, so that a slot keeps its XCD)
+  uint32_t grid = b->grid > 0 ? std::min<uint32_t>(b->grid, a.nslots) : a.nslots;
+  if (b->group > 1 && b->grid > 0) grid = std::max<uint32_t>(8u, grid / 8u * 8u);
   const Decoded& d = b->spec;
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipError_t e = g_batch_tables[d.storage][d.collision][d.obstacle][d.side][report ? 1 : 0](

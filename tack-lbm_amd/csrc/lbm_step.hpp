@@ -954,19 +954,34 @@ struct BatchArgs {
   uint32_t nchunks;
   tlbm_fields fld;   // packed over all domains (cell_off), or all null
   double* partials;  // report: [TLBM_NSUMS][nchunks]
+  uint32_t group;    // chunks per XCD run (1: plain order); slots = nchunks rounded up to 8 group
+  uint32_t nslots;
 };
+
+// The chunk of slot s: with group g > 1, the blocks an XCD runs (slot s goes to XCD s mod 8)
+// take runs of g consecutive chunks, so the cache lines two neighbouring chunks share (their
+// pulls' row and block edges) are fetched into one L2; g = 1 is plain order.  Slots past the
+// last chunk map past it.
+__device__ __forceinline__ uint32_t batch_chunk(uint32_t s, uint32_t g) {
+  if (g <= 1) return s;
+  const uint32_t x = s & 7u, j = s >> 3;
+  return g * (8u * (j / g) + x) + j % g;
+}
 
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool NT = false>
 __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
   using T = typename StorageT<STORAGE>::T;
   const T* __restrict__ fin = static_cast<const T*>(b.fin);
   T* __restrict__ fout = static_cast<T*>(b.fout);
-  // Chunks in plain order: consecutive chunks (neighbouring rows) go to consecutive XCDs, so
-  // the eight XCDs stream through nearby addresses of every plane together.  Handing each XCD
-  // a contiguous eighth of the chunks instead (to keep the x +- 1 edge lines of consecutive
-  // chunks in one L2) measured 7-9% slower at FP32 on MI355X: eight streams per plane, each
-  // an eighth of the lattice apart, contend for the same HBM channels.
-  for (uint32_t c = blockIdx.x; c < b.nchunks; c += gridDim.x) {
+  // Chunk order (batch_chunk): plain, consecutive chunks on consecutive XCDs, so the eight
+  // XCDs stream through nearby addresses of every plane together; or runs of `group`
+  // consecutive chunks per XCD, so the lines neighbouring chunks share stay in one L2.
+  // Handing each XCD a contiguous eighth of all chunks measured 7-9% slower at FP32 on
+  // MI355X: eight streams per plane, each an eighth of the lattice apart, contend for the
+  // same HBM channels.
+  for (uint32_t slot = blockIdx.x; slot < b.nslots; slot += gridDim.x) {
+    const uint32_t c = batch_chunk(slot, b.group);
+    if (c >= b.nchunks) continue;  // uniform over the block
     const BatchDomain& D = b.dom[b.chunk_dom[c]];
     const StepArgs& a = D.a;
     const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * a.nz;

@@ -91,6 +91,7 @@ SIGNATURES = {
                                      _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64),
                                      _c.POINTER(_c.c_uint64)]),
     "tlbm_batch_set_launch": (_c.c_int, [_vp, _c.c_int, _c.c_int]),
+    "tlbm_batch_set_order": (_c.c_int, [_vp, _c.c_int]),
     "tlbm_batch_init": (_c.c_int, [_vp, _vp, _c.c_float, _c.c_float, _c.c_uint64, _vp]),
     "tlbm_batch_step": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _c.POINTER(Fields), _vp, _vp]),
     "tlbm_batch_graph": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _c.c_int, _c.POINTER(Fields), _vp,

@@ -67,7 +67,8 @@ def _time(launch, stream, reps):
 
 
 def tune_batch(batch, reps=5, use_cache=True, candidates=None):
-    """Pick (threads per block, persistent grid) for a BatchSolver; returns (block, grid, ms)."""
+    """Pick (threads per block, persistent grid, chunk order) for a BatchSolver; applies them and
+    returns (block, grid, ms); the chunk order chosen is ``batch.group``."""
     L = lib()
     cus = ctypes.c_int()
     check(L.tlbm_device_cus(ctypes.byref(cus)), "device CUs")
@@ -75,8 +76,10 @@ def tune_batch(batch, reps=5, use_cache=True, candidates=None):
            + ",".join(f"{d.nx}x{d.ny}x{d.nz}" for d in batch.domains[:4]))
     table = _load() if use_cache else {}
     if key in table:
-        block, grid, ms = table[key]
+        block, grid, ms = table[key][:3]
+        group = table[key][3] if len(table[key]) > 3 else 1
         batch.set_launch(block, grid)
+        batch.set_order(group)
         return block, grid, ms
     if candidates is None:
         candidates = []
@@ -84,22 +87,32 @@ def tune_batch(batch, reps=5, use_cache=True, candidates=None):
             waves = block // 64
             for per_simd in (0, 2, 4, 8):  # 0: one block per chunk (not persistent)
                 grid = 0 if per_simd == 0 else cus.value * 4 * per_simd // waves
-                candidates.append((block, grid))
-    best = None
-    for block, grid in candidates:
-        batch.set_launch(block, grid)
+                for group in (1, 4, 8):
+                    candidates.append((block, grid, group))
+    def launch():
+        check(L.tlbm_batch_step(batch._batch, batch.f[0].ptr, batch.f[1].ptr, 0, None, None,
+                                batch.stream), "batch step")
 
-        def launch():
-            check(L.tlbm_batch_step(batch._batch, batch.f[0].ptr, batch.f[1].ptr, 0, None, None,
-                                    batch.stream), "batch step")
-        ms = _time(launch, batch.stream, reps)
+    def timed(cand, n):
+        batch.set_launch(cand[0], cand[1])
+        batch.set_order(cand[2] if len(cand) > 2 else 1)
+        return _time(launch, batch.stream, n)
+
+    # a short pass over every candidate, then the six fastest again with four times the steps
+    # (candidates within a few percent are told apart only by the longer timing)
+    first = sorted((timed(c, reps), i) for i, c in enumerate(candidates))
+    best = None
+    for _, i in first[:6]:
+        cand = candidates[i]
+        ms = timed(cand, 4 * reps)
         if best is None or ms < best[2]:
-            best = (block, grid, ms)
+            best = (cand[0], cand[1], ms, cand[2] if len(cand) > 2 else 1)
     batch.set_launch(best[0], best[1])
+    batch.set_order(best[3])
     if use_cache:
         table[key] = list(best)
         _store(table)
-    return best
+    return best[:3]
 
 
 def tune_solver(solver, reps=5, use_cache=True, blocks=(64, 128, 192, 256)):

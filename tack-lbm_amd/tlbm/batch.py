@@ -109,6 +109,7 @@ class BatchSolver:
         check(lib().tlbm_batch_init(self._batch, self.f[0].ptr, seed_amp, noise_amp, noise_seed,
                                     stream), "batch init")
         self.step_count = 0
+        self.group = 1
         self._graphs = {}
 
     def _layout(self, d):
@@ -129,6 +130,15 @@ class BatchSolver:
         self._graphs.clear()
         check(lib().tlbm_batch_set_launch(self._batch, int(block_x), int(grid_blocks)),
               "batch set launch")
+
+    def set_order(self, group=1):
+        """Chunk order: 1 plain, g > 1 runs of g consecutive chunks per XCD (tlbm_batch_set_order).
+        Captured graphs hold the order they were captured with, so they are dropped."""
+        for g in self._graphs.values():
+            lib().tlbm_graph_destroy(g)
+        self._graphs.clear()
+        check(lib().tlbm_batch_set_order(self._batch, int(group)), "batch set order")
+        self.group = int(group)
 
     def step(self, report=False):
         """One step of every domain (lbm.py:639-647 for all of them at once)."""

@@ -103,15 +103,18 @@ def test_batch_step_matches_single_lattice_and_oracle(storage, collision, plane_
 
 @pytest.mark.parametrize("block,grid", [(64, 0), (128, 0), (256, 0), (64, 7), (256, 3),
                                         (128, 1000), (192, 0), (192, 5)])
-def test_batch_launch_shapes_agree(block, grid):
-    """Quiet and report steps give the same state and fields at every launch shape, and the
-    same per-domain FP64 sums up to summation order."""
+@pytest.mark.parametrize("group", [1, 3, 8])
+def test_batch_launch_shapes_agree(block, grid, group):
+    """Quiet and report steps give the same state and fields at every launch shape and chunk
+    order (runs of `group` consecutive chunks per XCD; persistent grids rounded to 8 blocks),
+    and the same per-domain FP64 sums up to summation order."""
     import tlbm
     from tlbm.batch import BatchSolver
     out = []
-    for shape in ((0, 0), (block, grid)):
+    for shape, order in (((0, 0), 1), ((block, grid), group)):
         b = BatchSolver(_domains(), collision="trt", storage="f32", fields=True)
         b.set_launch(*shape)
+        b.set_order(order)
         _random_state(b, seed=6)
         b.step()
         b.step(report=True)
@@ -136,6 +139,7 @@ def test_batch_graph_replays_match_eager_steps():
     tlbm._capi.check(L.tlbm_stream_create(ctypes.byref(stream)), "stream")
     g = BatchSolver(_domains(), collision="les", storage="f16", stream=stream.value,
                     noise_amp=1e-3)
+    g.set_order(4)  # graphs hold the chunk order they were captured with
     e = BatchSolver(_domains(), collision="les", storage="f16", noise_amp=1e-3)
     for replay in range(3):
         g.run(4, report_every=2)
