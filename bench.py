@@ -29,7 +29,7 @@ replays, under "graph".
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-sweep]
                        [--no-adjoint] [--no-derived] [--no-heat] [--no-loop] [--graph]
                        [--storage f32,f16[,bf16]] [--block N]
-                       [--calibrate]
+                       [--calibrate] [--sweep-steps S] [--sweep-launch BLOCK,GRID,GROUP]
 """
 
 import argparse
@@ -241,12 +241,13 @@ def calibrate_copy(gib=1, reps=5):
             "gbs": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
-def time_sweep(storage, steps=5000, graph_steps=100, seq_steps=20):
+def time_sweep(storage, steps=5000, graph_steps=100, seq_steps=20, launch=None):
     """The parameter sweep: 64 seeded TRT + Smagorinsky channels (nx in [64, 192], ny = nz =
     nx / 2, Re in [100, 1000], cylinder radius in [4, 12], u0 = 0.08, seeded 1e-3 noise) stepped
     `steps` times by one persistent launch per step, replayed from a hipGraph of `graph_steps`
     steps with a report step at its end; launch shape from the autotuner.  For comparison, the
-    same domains stepped one launch per domain (one Solver each) for `seq_steps` steps."""
+    same domains stepped one launch per domain (one Solver each) for `seq_steps` steps.
+    `launch` = (block, persistent grid, chunk group) pins the launch shape instead."""
     import tlbm
     from tlbm import autotune
     from tlbm.batch import BatchSolver, sweep_domains
@@ -257,7 +258,13 @@ def time_sweep(storage, steps=5000, graph_steps=100, seq_steps=20):
     check(L.tlbm_stream_create(ctypes.byref(stream)), "stream create")
     b = BatchSolver(domains, collision="trt", storage=storage, noise_amp=1e-3,
                     stream=stream.value)
-    block, grid, tune_ms = autotune.tune_batch(b)
+    if launch is None:
+        block, grid, tune_ms = autotune.tune_batch(b)
+    else:
+        block, grid, group = launch
+        b.set_launch(block, grid)
+        b.set_order(group)
+        tune_ms = 0.0
     group = b.group
     b.run(graph_steps, report_every=graph_steps)  # capture + warm-up
     ev = Events(L)
@@ -483,6 +490,13 @@ def cpu_baseline(max_seconds=15.0):
                       f"{usable} usable"}
 
 
+def _launch_triple(text):
+    parts = [int(v) for v in text.split(",")]
+    if len(parts) != 3:
+        raise argparse.ArgumentTypeError("expected BLOCK,GRID,GROUP")
+    return tuple(parts)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -492,6 +506,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="threads per block (0 = default)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched parameter sweep")
+    ap.add_argument("--sweep-steps", type=int, default=5000,
+                    help="timed steps of the sweep (a multiple of 100)")
+    ap.add_argument("--sweep-launch", type=_launch_triple, default=None,
+                    help="BLOCK,GRID,GROUP: the sweep's launch shape and chunk order, not tuned")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--no-derived", action="store_true", help="skip the derived-field timing")
     ap.add_argument("--no-heat", action="store_true", help="skip the temperature-variant timing")
@@ -506,6 +524,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is "
               "described, not built", file=sys.stderr)
         sys.exit(2)
+    if args.sweep_steps <= 0 or args.sweep_steps % 100:
+        ap.error("--sweep-steps must be a positive multiple of 100")
     storages = [s for s in args.storage.split(",") if s]
     results = {}
     for st in storages:
@@ -518,7 +538,7 @@ def main():
     sweep = {}
     if not args.no_sweep:
         for st in ("f32", "f16"):
-            sweep[st] = time_sweep(st)
+            sweep[st] = time_sweep(st, steps=args.sweep_steps, launch=args.sweep_launch)
             print(f"[bench] sweep {st}: {sweep[st]['mlups']:.0f} MLUPS over {sweep[st]['cells']} "
                   f"cells in {sweep[st]['domains']} domains (one launch per domain: "
                   f"{sweep[st]['one_launch_per_domain_mlups']:.0f})", file=sys.stderr, flush=True)
