@@ -188,7 +188,7 @@ def time_ceiling(s, stream, reps=50):
     import tlbm
     from tlbm._capi import check, lib
     L = lib()
-    p = tlbm.make_params(s.lat, s.phys, s.geo, s.flow, s.qstride)
+    p = tlbm.make_params(s.lat, s.phys, s.geo, s.flow, s.qstride, s.tile_rows)
     code = tlbm.STORAGES[s.storage]
     for _ in range(5):
         check(L.tlbm_pull_copy(s.f[0].ptr, s.f[1].ptr, ctypes.byref(p), code, stream), "pull copy")

@@ -13,7 +13,12 @@
  *   - pointers to lattice data are device (or managed) pointers; "stream" is a hipStream_t
  *     passed as void* (NULL = the null stream).  All launches are asynchronous.
  *   - layout is the reference's: population q of cell idx = (z*ny + y)*nx + x lives at
- *     f[q*qstride + idx] (lbm.py:386-388, 426, 456), with qstride >= nx*ny*nz.
+ *     f[q*qstride + idx] (lbm.py:386-388, 426, 456), with qstride >= nx*ny*nz.  A step can
+ *     instead take populations in tiles of k = tile_rows rows (tlbm_params.tile_rows > 0, no
+ *     counterpart in the reference): with row r = z*ny + y and T = k*nx, population q of cell
+ *     (x, r) lives at f[(r/k)*19*T + q*T + (r%k)*nx + x], i.e. each tile of k rows holds its 19
+ *     planes together; tlbm_population_elems gives the buffer length.  The temperature planes
+ *     g stay q-major with qstride in both layouts.
  */
 #ifndef TLBM_H
 #define TLBM_H
@@ -68,7 +73,10 @@ typedef struct {
   int32_t nx, ny, nz;      /* D3Q19(nx, ny, nz), lbm.py:42-44                           */
   int32_t block_x;         /* threads per block of the step: 64, 128, 192 or 256; 0: the
                               handle's default (tlbm_set_launch, else 256)                */
-  int64_t qstride;         /* elements between population planes, >= nx*ny*nz           */
+  int64_t qstride;         /* elements between population planes, >= nx*ny*nz (with
+                              tile_rows > 0: the temperature planes only)                 */
+  int32_t tile_rows;       /* 0: q-major population planes; k > 0 (a power of two): tiles
+                              of k rows, 19 planes of k*nx cells each (see the layout note) */
   float omega;             /* BGK: 1 / (3 nu + 1/2), lbm.py:63                          */
   float tau0;              /* TRT / LES: 3 nu + 1/2, lbm.py:110, 131                    */
   float cs;                /* Smagorinsky constant (TRT.cs, SmagorinskyLES.cs = 0.16)    */
@@ -127,6 +135,9 @@ int tlbm_kernel_lookup(const tlbm_spec* spec, int* handle);
 /* Launch shape of a specialisation: threads per block (64, 128, 192 or 256; 0 restores the
  * default, 256).  Each block steps that many consecutive cells in index order. */
 int tlbm_set_launch(int handle, int block_x);
+/* Elements one population buffer must hold for `params` (nx, ny, nz, qstride, tile_rows):
+ * 19*qstride for q-major planes, 19*k*nx*ceil(ny*nz/k) for tiles of k rows. */
+int tlbm_population_elems(const tlbm_params* params, uint64_t* elems);
 
 /* lbm_init (lbm.py:383-394): f = feq(1, u0 fluid, seed_amp u0 fluid sin(2 pi x/nx)
  * cos(2 pi z/nz), 0), plus, when noise_amp != 0, a seeded uniform U(-noise_amp, noise_amp)

@@ -265,6 +265,14 @@ __global__ void __launch_bounds__(256) copy16_kernel(const uint4* __restrict__ s
   if (i < n16) dst[i] = src[i];
 }
 
+// Elements of one population buffer: 19 planes of qstride, or 19 planes of k nx per tile of k rows.
+uint64_t population_elems(const tlbm_params& p) {
+  if (p.tile_rows <= 0) return 19ull * static_cast<uint64_t>(p.qstride);
+  const uint64_t k = static_cast<uint64_t>(p.tile_rows);
+  const uint64_t rows = static_cast<uint64_t>(p.ny) * static_cast<uint64_t>(p.nz);
+  return 19ull * k * static_cast<uint64_t>(p.nx) * ((rows + k - 1) / k);
+}
+
 int check_params(const tlbm_params* p, int obstacle = -1) {
   if (!p) return fail(TLBM_ERR_INVALID, "params is NULL");
   if (obstacle == TLBM_FIELD && !p->phi)
@@ -282,6 +290,11 @@ int check_params(const tlbm_params* p, int obstacle = -1) {
                 static_cast<unsigned long long>(n));
   if (static_cast<uint64_t>(p->qstride) * 18 + n > (1ull << 40))
     return fail(TLBM_ERR_INVALID, "population buffer too large");
+  if (p->tile_rows < 0 || p->tile_rows > (1 << 20) || (p->tile_rows & (p->tile_rows - 1)))
+    return fail(TLBM_ERR_INVALID, "tile_rows %d is not 0 or a power of two up to 2^20",
+                p->tile_rows);
+  if (population_elems(*p) > (1ull << 40))
+    return fail(TLBM_ERR_INVALID, "population buffer too large");
   return TLBM_OK;
 }
 
@@ -298,25 +311,39 @@ bool stream_stores(uint64_t elements, int storage) {
 // elem_bytes: bytes per stored population (4 for FP32, 2 for FP16/BF16).
 tlbm::StepArgs make_args(const tlbm_params& p, int obstacle, uint32_t elem_bytes) {
   tlbm::StepArgs a{};
-  a.ns = static_cast<uint64_t>(p.qstride);
+  const bool tiled = p.tile_rows > 0;
+  a.ns = tiled ? static_cast<uint64_t>(p.tile_rows) * static_cast<uint64_t>(p.nx)
+               : static_cast<uint64_t>(p.qstride);
+  a.gns = static_cast<uint64_t>(p.qstride);
+  a.tile_stride = tiled ? 19ull * a.ns : 0ull;
+  a.tile_log2 = 31u;
+  a.tile_mask = 0xFFFFFFFFu;
+  if (tiled) {
+    uint32_t l = 0;
+    while ((1 << l) < p.tile_rows) ++l;
+    a.tile_log2 = l;
+    a.tile_mask = static_cast<uint32_t>(p.tile_rows - 1);
+  }
+  a.row_waves = p.nx % 64 == 0 ? 1u : 0u;
   a.nx = p.nx;
   a.ny = p.ny;
   a.nz = p.nz;
   // Buffer-resource addressing (lbm_step.hpp) needs every plane offset plus every lane offset
-  // below 2^32 bytes: the last plane's start (18 qstride) plus a cell index and one row.
+  // below 2^32 bytes: the buffer's length plus a cell index and one row.
   const uint64_t cells = static_cast<uint64_t>(p.nx) * p.ny * p.nz;
-  const uint64_t span = 19ull * static_cast<uint64_t>(p.qstride) + cells +
-                        static_cast<uint64_t>(p.nx);
+  const uint64_t span = population_elems(p) + cells + static_cast<uint64_t>(p.nx);
   a.off32 = span * elem_bytes < (1ull << 32) ? 1u : 0u;
   if (a.off32) {
     const int64_t plane = static_cast<int64_t>(p.nx) * p.ny;
+    const int64_t ns = static_cast<int64_t>(a.ns);
     for (int q = 0; q < tlbm::kQ; ++q) {
       const int64_t shift = tlbm::kCX[q] + tlbm::kCY[q] * static_cast<int64_t>(p.nx) +
                             tlbm::kCZ[q] * plane;
-      a.plane_off[q] = static_cast<uint32_t>(q * p.qstride * elem_bytes);
-      // plane q's start less its pull shift: >= 0 for q >= 1 since qstride >= cells >= the
-      // shift; q = 0 has no shift
-      a.pull_off[q] = static_cast<uint32_t>((q * p.qstride - shift) * elem_bytes);
+      a.plane_off[q] = static_cast<uint32_t>(q * ns * elem_bytes);
+      // q-major planes: plane q's start less its pull shift, >= 0 for q >= 1 since
+      // qstride >= cells >= the shift; q = 0 has no shift.  (Tiles add the source row's offset
+      // to plane_off in the kernel instead.)
+      a.pull_off[q] = tiled ? 0u : static_cast<uint32_t>((q * ns - shift) * elem_bytes);
     }
   }
   a.div_nx = tlbm::make_divisor(static_cast<uint32_t>(p.nx));
@@ -673,6 +700,14 @@ int tlbm_set_launch(int handle, int block_x) {
   return TLBM_OK;
 }
 
+int tlbm_population_elems(const tlbm_params* params, uint64_t* elems) {
+  if (!elems) return fail(TLBM_ERR_INVALID, "elems is NULL");
+  *elems = 0;
+  if (int rc = check_params(params)) return rc;
+  *elems = population_elems(*params);
+  return TLBM_OK;
+}
+
 int tlbm_init(int handle, void* f, const tlbm_params* params, float seed_amp, float noise_amp,
               uint64_t noise_seed, void* stream) {
   std::call_once(g_once, build_tables);
@@ -711,7 +746,8 @@ int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const
   a.fin = f_in;
   a.fout = f_out;
   // the temperature step adds 7 FP32 planes (g) to the 19 population planes
-  a.stream_stores = stream_stores((heat ? 26 : 19) * static_cast<uint64_t>(params->qstride),
+  a.stream_stores = stream_stores(population_elems(*params) +
+                                      (heat ? 7 * static_cast<uint64_t>(params->qstride) : 0),
                                   heat ? TLBM_F32 : d.storage) ? 1u : 0u;
   if (heat) {
     if (!g_in || !g_out) return fail(TLBM_ERR_INVALID, "g_in or g_out is NULL");
@@ -844,7 +880,7 @@ int tlbm_pull_copy(const void* f_in, void* f_out, const tlbm_params* params, int
   tlbm::StepArgs a = make_args(*params, TLBM_CYLINDER, storage == TLBM_F32 ? 4u : 2u);
   a.fin = f_in;
   a.fout = f_out;
-  const bool nt = stream_stores(19 * static_cast<uint64_t>(params->qstride), storage);
+  const bool nt = stream_stores(population_elems(*params), storage);
   const uint64_t n = static_cast<uint64_t>(params->nx) * params->ny * params->nz;
   const uint64_t nblocks = (n + 255) / 256;
   if (nblocks * 256 > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
@@ -895,6 +931,9 @@ int tlbm_batch_create(int handle, const tlbm_params* params, int ndomains, tlbm_
     const int64_t n = static_cast<int64_t>(p.nx) * p.ny * p.nz;
     if (p.qstride < n) p.qstride = n;  // a plane stride below the cell count means "packed"
     if (int rc = check_params(&p, d.obstacle)) return rc;
+    if (p.tile_rows != 0)
+      return fail(TLBM_ERR_INVALID, "batched domains use q-major planes (domain %d has tile_rows %d)",
+                  i, p.tile_rows);
   }
   tlbm_batch* b = new tlbm_batch();
   b->handle = handle - d.diag;
@@ -1107,6 +1146,8 @@ int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* 
                       const int32_t* block_origin, const int32_t* block_size, double* gphi,
                       int final_links, void* stream) {
   if (int rc = check_params(params, TLBM_FIELD)) return rc;
+  if (params->tile_rows != 0)
+    return fail(TLBM_ERR_INVALID, "the adjoint step takes q-major population planes (tile_rows 0)");
   if (!f_t1 || !mu_in || (!final_links && (!f_t || !mu_out)))
     return fail(TLBM_ERR_INVALID, "adjoint buffers are NULL");
   if (!final_links && mu_in == mu_out)

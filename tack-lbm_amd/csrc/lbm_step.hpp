@@ -168,15 +168,23 @@ __host__ __device__ __forceinline__ uint32_t divide(uint32_t n, const Divisor& d
 struct StepArgs {
   const void* fin;
   void* fout;
-  uint64_t ns;               // plane stride (elements)
+  uint64_t ns;               // population plane stride (elements): qstride, or k nx in tiles
+  uint64_t gns;              // temperature plane stride (elements): qstride
+  // Population layout.  Row r = z ny + y starts at element row_elem(r) of plane 0:
+  //   q-major planes: tile_stride = 0, tile_log2 = 31, tile_mask = all ones -> r nx;
+  //   tiles of k rows: tile_stride = 19 k nx, tile_log2 = log2 k, tile_mask = k - 1 ->
+  //   (r / k) 19 k nx + (r % k) nx, each tile holding its 19 planes of k nx cells together.
+  uint64_t tile_stride;
+  uint32_t tile_log2, tile_mask;
+  uint32_t row_waves;        // 1: nx is a multiple of 64, so each wave of 64 cells is in one row
   int32_t nx, ny, nz;
   int32_t wrap;              // bit 0/1/2: the obstacle reaches across the x/y/z periodic faces
   uint32_t stream_stores;    // 1: launch the non-temporal-store (NT) quiet kernel (host side)
   uint32_t off32;            // 1: the whole population buffer lies within 2^32 bytes of its
                              // start, so it is addressed as one buffer resource with 32-bit
                              // plane (scalar) and lane (vector) offsets
-  uint32_t pull_off[19];     // off32: byte offset of plane q less the pull shift
-                             // (c_x + c_y nx + c_z nx ny) of population q
+  uint32_t pull_off[19];     // off32, q-major planes: byte offset of plane q less the pull
+                             // shift (c_x + c_y nx + c_z nx ny) of population q
   uint32_t plane_off[19];    // off32: byte offset of plane q
   float omega;               // BGK
   float tau0, smag, magic;   // smag = 25.456 cs^2 (lbm.py:93)
@@ -199,6 +207,12 @@ struct StepArgs {
   float* temp;
   float alpha, nu_mol, pr_t;
 };
+
+// First element of row r (= z ny + y) in population plane 0 (see StepArgs).
+__device__ __forceinline__ uint64_t row_elem(const StepArgs& a, uint32_t r) {
+  return static_cast<uint64_t>(r >> a.tile_log2) * a.tile_stride +
+         (r & a.tile_mask) * static_cast<uint32_t>(a.nx);
+}
 
 // ---- obstacles: lbm.py:144-227 ---------------------------------------------------------------
 // solid() uses one rounding per subtract, multiply and add (no contraction), as a float32
@@ -526,8 +540,8 @@ __device__ __forceinline__ void pull_raw(const StepArgs& a, const T* __restrict_
                                          int z, T (&raw)[kQ]) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const uint32_t snx = static_cast<uint32_t>(nx);
-  const uint32_t plane = snx * static_cast<uint32_t>(ny);
-  const uint32_t idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
+  const uint32_t row0 = static_cast<uint32_t>(z) * ny + y;
+  const uint32_t idx = row0 * snx + x;
   const int xsrc[3] = {x == nx - 1 ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};
   const int zsrc[3] = {z == nz - 1 ? 0 : z + 1, z, z == 0 ? nz - 1 : z - 1};
   int ysrc[3];
@@ -553,18 +567,37 @@ __device__ __forceinline__ void pull_raw(const StepArgs& a, const T* __restrict_
   // general per-lane path.  FP32 keeps the general path everywhere: it is bandwidth-bound, and
   // its buffer loads measured 2% slower than per-lane global loads (0.912 against 0.893 ms at
   // 512x256x256 in one process, same box).
+  // Tiles of rows: when each wave lies in one row (row_waves), the source row of population q
+  // is the same for the whole wave, so its offset (tile, plane, row in the tile) is computed in
+  // scalar registers and the lanes add only their source column.
   const bool inner = y >= 1 && y < ny - 1 && z >= 1 && z < nz - 1;
-  if (sizeof(T) == 2 && a.off32 && __builtin_amdgcn_ballot_w64(!inner) == 0) {
+  if (sizeof(T) == 2 && a.off32 && (a.tile_stride == 0 || a.row_waves) &&
+      __builtin_amdgcn_ballot_w64(!inner) == 0) {
     constexpr uint32_t B = sizeof(T);
     const auto rin = plane_rsrc(fin);
-    const uint32_t v0 = idx * B;
-    const uint32_t vp = (idx + (x == 0 ? snx : 0u)) * B;       // c_x = +1: x - 1, wrapped at 0
-    const uint32_t vm = (idx - (x == nx - 1 ? snx : 0u)) * B;  // c_x = -1: x + 1, wrapped
-    static_for<0, kQ>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      constexpr int cx = kCX[q];
-      raw[q] = buffer_load<T>(rin, cx == 1 ? vp : (cx == -1 ? vm : v0), a.pull_off[q]);
-    });
+    if (a.tile_stride == 0) {
+      const uint32_t v0 = idx * B;
+      const uint32_t vp = (idx + (x == 0 ? snx : 0u)) * B;       // c_x = +1: x - 1, wrapped at 0
+      const uint32_t vm = (idx - (x == nx - 1 ? snx : 0u)) * B;  // c_x = -1: x + 1, wrapped
+      static_for<0, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = kCX[q];
+        raw[q] = buffer_load<T>(rin, cx == 1 ? vp : (cx == -1 ? vm : v0), a.pull_off[q]);
+      });
+    } else {
+      const uint32_t r0 = __builtin_amdgcn_readfirstlane(row0);
+      const uint32_t ts = static_cast<uint32_t>(a.tile_stride);
+      const uint32_t v0 = static_cast<uint32_t>(x) * B;
+      const uint32_t vp = static_cast<uint32_t>(xsrc[2]) * B;  // c_x = +1
+      const uint32_t vm = static_cast<uint32_t>(xsrc[0]) * B;  // c_x = -1
+      static_for<0, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = kCX[q];
+        const uint32_t r = r0 - static_cast<uint32_t>(kCY[q] + kCZ[q] * ny);  // no face: no wrap
+        const uint32_t so = ((r >> a.tile_log2) * ts + (r & a.tile_mask) * snx) * B + a.plane_off[q];
+        raw[q] = buffer_load<T>(rin, cx == 1 ? vp : (cx == -1 ? vm : v0), so);
+      });
+    }
   } else {
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
@@ -572,9 +605,9 @@ __device__ __forceinline__ void pull_raw(const StepArgs& a, const T* __restrict_
       int qs = q;
       if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = mir_lo ? miry(q) : q;
       if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = mir_hi ? miry(q) : q;
-      const T* __restrict__ row = fin + static_cast<uint64_t>(qs) * a.ns +
-                                  static_cast<uint32_t>(zsrc[1 + cz]) * plane +
-                                  static_cast<uint32_t>(ysrc[1 + cy]) * snx;
+      const T* __restrict__ row =
+          fin + static_cast<uint64_t>(qs) * a.ns +
+          row_elem(a, static_cast<uint32_t>(zsrc[1 + cz]) * ny + static_cast<uint32_t>(ysrc[1 + cy]));
       raw[q] = row[xsrc[1 + cx]];
     });
   }
@@ -584,21 +617,23 @@ __device__ __forceinline__ void pull_raw(const StepArgs& a, const T* __restrict_
 // NT: non-temporal stores, for lattices whose two population buffers are far larger than the
 // Infinity Cache (measured on MI355X: 1.6% faster at FP32, 3.4% at FP16, 512x256x256); below
 // that size the next step's loads hit the cache lines plain stores leave there.
+// own: the cell's element in plane 0 (row_elem of its row + x).
 template <bool NT, class T>
-__device__ __forceinline__ void store_raw(const StepArgs& a, T* __restrict__ fout, uint32_t idx,
+__device__ __forceinline__ void store_raw(const StepArgs& a, T* __restrict__ fout, uint64_t own,
                                           const T (&raw)[kQ]) {
   if (sizeof(T) == 2 && a.off32) {
     // one lane offset for all 19 planes, the plane offsets in scalar registers
     constexpr uint32_t B = sizeof(T);
     const auto rout = plane_rsrc(fout);
+    const uint32_t vo = static_cast<uint32_t>(own) * B;
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      buffer_store<T, NT>(raw[q], rout, idx * B, a.plane_off[q]);
+      buffer_store<T, NT>(raw[q], rout, vo, a.plane_off[q]);
     });
   } else {
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      T* const dst = (fout + static_cast<uint64_t>(q) * a.ns) + idx;
+      T* const dst = (fout + static_cast<uint64_t>(q) * a.ns) + own;
       if constexpr (NT) __builtin_nontemporal_store(raw[q], dst);
       else *dst = raw[q];
     });
@@ -633,7 +668,7 @@ __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z
     constexpr int cx = kCX7[q], cy = kCY7[q], cz = kCZ7[q];
     nidx[q] = static_cast<uint32_t>(zs[1 + cz]) * plane + static_cast<uint32_t>(ys[1 + cy]) * snx +
               static_cast<uint32_t>(xs[1 + cx]);
-    g[q] = gin[static_cast<uint64_t>(q) * a.ns + nidx[q]];
+    g[q] = gin[static_cast<uint64_t>(q) * a.gns + nidx[q]];
   });
   const bool wrap = ((a.wrap & 1) && (x == 0 || x == nx - 1)) ||
                     ((a.wrap & 2) && (y == 0 || y == ny - 1)) ||
@@ -647,7 +682,7 @@ __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z
     static_for<0, kQ7>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = kCX7[q], cy = kCY7[q], cz = kCZ7[q];
-      own[q] = gin[static_cast<uint64_t>(opp7(q)) * a.ns + idx];
+      own[q] = gin[static_cast<uint64_t>(opp7(q)) * a.gns + idx];
       wall[q] = q == 0 ? here : Obs::solid(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]);
       hot[q] = wall[q] != 0.0f ? Obs::sink(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]) : 0.0f;
     });
@@ -677,7 +712,7 @@ __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z
       const float eq = kW7[q] * temp * (1.0f + 4.0f * (cx * ux + cy * uy + cz * uz));
       const float post = g[q] - omega_g * (g[q] - eq);
       const float eqs = kW7[q] * temp_s * (1.0f + 4.0f * (cx * ex.uxs + cy * ex.uys + cz * ex.uzs));
-      float* dst = &a.g_out[static_cast<uint64_t>(q) * a.ns + idx];
+      float* dst = &a.g_out[static_cast<uint64_t>(q) * a.gns + idx];
       if constexpr (NT) __builtin_nontemporal_store(eqs + keep * (post - eq), dst);
       else *dst = eqs + keep * (post - eq);
     });
@@ -713,8 +748,9 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   constexpr bool kShift = St::kShift;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const uint32_t snx = static_cast<uint32_t>(nx);
-  const uint32_t plane = snx * static_cast<uint32_t>(ny);
   const uint32_t idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
+  // the cell's own element in population plane 0 (idx for q-major planes)
+  const uint64_t own_e = row_elem(a, static_cast<uint32_t>(z) * ny + y) + static_cast<uint32_t>(x);
 
   // pull sources (x, z periodic: lbm.py:412, 415; y by side: lbm.py:342-369), indexed [1 + c]:
   // the source of a population moving with +c is one cell back.
@@ -753,7 +789,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
           constexpr int o = opp(q);
           const float t =
               6.0f * kW[q] * (kCX[q] * a.lid[0] + kCY[q] * a.lid[1] + kCZ[q] * a.lid[2]);
-          f[q] = St::value(fin[static_cast<uint64_t>(o) * a.ns + idx]) + t;
+          f[q] = St::value(fin[static_cast<uint64_t>(o) * a.ns + own_e]) + t;
         }
       });
     }
@@ -779,12 +815,13 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
       const auto rin = plane_rsrc(fin);
       static_for<1, kQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        own[q] = buffer_load<T>(rin, idx * static_cast<uint32_t>(sizeof(T)), a.plane_off[opp(q)]);
+        own[q] = buffer_load<T>(rin, static_cast<uint32_t>(own_e) * static_cast<uint32_t>(sizeof(T)),
+                                a.plane_off[opp(q)]);
       });
     } else {
       static_for<1, kQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        own[q] = (fin + static_cast<uint64_t>(opp(q)) * a.ns)[idx];
+        own[q] = (fin + static_cast<uint64_t>(opp(q)) * a.ns)[own_e];
       });
     }
     static_for<1, kQ>([&](auto qc) {
@@ -829,7 +866,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   if (active) {
     T raw[kQ];
     static_for<0, kQ>([&](auto qc) { raw[decltype(qc)::value] = St::encode(f[decltype(qc)::value]); });
-    store_raw<NT>(a, fout, idx, raw);
+    store_raw<NT>(a, fout, own_e, raw);
   }
   if constexpr (HEAT) heat_cell<OBS, REPORT, NT>(a, x, y, z, idx, here, ux, uy, uz, ex, active);
 
@@ -930,7 +967,7 @@ __global__ void __launch_bounds__(256) pull_copy_kernel(const StepArgs a) {
   T raw[kQ];
   pull_raw<TLBM_PERIODIC_Y>(a, static_cast<const T*>(a.fin), static_cast<int>(x),
                             static_cast<int>(y), static_cast<int>(z), raw);
-  store_raw<NT>(a, static_cast<T*>(a.fout), idx, raw);
+  store_raw<NT>(a, static_cast<T*>(a.fout), row_elem(a, z * a.ny + y) + x, raw);
 }
 
 // ---- batched lattices: many independent domains in one persistent launch -------------------------
@@ -1040,9 +1077,10 @@ __global__ void __launch_bounds__(256) lbm_init_kernel(const StepArgs a, float s
   }
   float e[kQ];
   equilibria<St::kShift>(1.0f, ux, uy, uz, e);
+  const uint64_t own_e = row_elem(a, zr * a.ny + yr) + xr;
   static_for<0, kQ>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    f[static_cast<uint64_t>(q) * a.ns + idx] = St::encode(e[q]);
+    f[static_cast<uint64_t>(q) * a.ns + own_e] = St::encode(e[q]);
   });
 }
 

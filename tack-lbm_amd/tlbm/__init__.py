@@ -11,5 +11,6 @@ from .lbm import (BGK, COLLISIONS, CX, CY, CZ, FIELDS, STORAGES, TRT, W, Box,  #
                   Cylinder, D3Q19, DeviceArray, DeviceField, Flow, Heat, HeatSink, HostBuffer,
                   LidY, ManagedArray, NoHeat, OutputArray, PeriodicY, Quiet, Report, SlipY,
                   SmagorinskyLES, SolidField, Solver, Sphere, array, coarsen, decode_populations,
-                  empty, encode_populations, kernel_handle, lbm_init, lbm_step, make_params,
-                  set_launch, sync, zeros)
+                  default_tile_rows, empty, encode_populations, kernel_handle, lbm_init,
+                  lbm_step, make_params, population_elems, set_launch, sync, tile_populations,
+                  untile_populations, zeros)

@@ -30,7 +30,7 @@ class Spec(ctypes.Structure):
 class Params(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
                 ("block_x", ctypes.c_int32), ("qstride", ctypes.c_int64),
-                ("omega", ctypes.c_float), ("tau0", ctypes.c_float), ("cs", ctypes.c_float),
+                ("tile_rows", ctypes.c_int32), ("omega", ctypes.c_float), ("tau0", ctypes.c_float), ("cs", ctypes.c_float),
                 ("magic", ctypes.c_float), ("obs", ctypes.c_float * 8), ("u0", ctypes.c_float),
                 ("sponge_start", ctypes.c_float), ("sponge_strength", ctypes.c_float),
                 ("lid", ctypes.c_float * 3), ("omega_rot", ctypes.c_void_p),
@@ -77,6 +77,7 @@ SIGNATURES = {
     "tlbm_event_elapsed_ms": (_c.c_int, [_vp, _vp, _c.POINTER(_c.c_float)]),
     "tlbm_kernel_lookup": (_c.c_int, [_c.POINTER(Spec), _c.POINTER(_c.c_int)]),
     "tlbm_set_launch": (_c.c_int, [_c.c_int, _c.c_int]),
+    "tlbm_population_elems": (_c.c_int, [_c.POINTER(Params), _c.POINTER(_c.c_uint64)]),
     "tlbm_init": (_c.c_int, [_c.c_int, _vp, _c.POINTER(Params), _c.c_float, _c.c_float,
                              _c.c_uint64, _vp]),
     "tlbm_step": (_c.c_int, [_c.c_int, _vp, _vp, _c.POINTER(Params), _c.POINTER(Fields),

@@ -129,7 +129,8 @@ def tune_solver(solver, reps=5, use_cache=True, blocks=(64, 128, 192, 256)):
         solver.set_launch(block)
         return block, ms
     from .lbm import make_params
-    params = make_params(lat, solver.phys, solver.geo, solver.flow, solver.qstride)
+    params = make_params(lat, solver.phys, solver.geo, solver.flow, solver.qstride,
+                         solver.tile_rows)
     best = None
     for block in blocks:
         solver.set_launch(block)

@@ -41,9 +41,9 @@ def test_struct_layout_matches_the_c_compiler(tmp_path):
 #include <stddef.h>
 #include "tlbm.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(tlbm_params), offsetof(tlbm_params, qstride),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(tlbm_params), offsetof(tlbm_params, qstride),
          offsetof(tlbm_params, obs), offsetof(tlbm_params, lid), offsetof(tlbm_params, omega_rot),
-         sizeof(tlbm_spec), sizeof(tlbm_fields));
+         sizeof(tlbm_spec), sizeof(tlbm_fields), offsetof(tlbm_params, tile_rows));
   return 0;
 }
 """)
@@ -54,7 +54,7 @@ int main(void) {
                                           check=True).stdout.split()]
     P = _capi.Params
     expect = [ctypes.sizeof(P), P.qstride.offset, P.obs.offset, P.lid.offset, P.omega_rot.offset,
-              ctypes.sizeof(_capi.Spec), ctypes.sizeof(_capi.Fields)]
+              ctypes.sizeof(_capi.Spec), ctypes.sizeof(_capi.Fields), P.tile_rows.offset]
     assert got == expect
 
 

@@ -199,7 +199,7 @@ def test_large_batch_domain_steps_as_the_single_lattice():
                     qstride=b.plane_strides[1])
     # the Solver takes one report step at construction (lbm.py:583-584); restart it from the
     # batch's initial state
-    tlbm.lbm_init(s.lat, s.geo, s.flow, s.f[0], 0.05, 1e-3, 0)
+    tlbm.lbm_init(s.lat, s.geo, s.flow, s.f[0], 0.05, 1e-3, 0, tile_rows=s.tile_rows)
     s.step()
     tlbm.sync()
     ref = s.populations()

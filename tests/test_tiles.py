@@ -1,0 +1,98 @@
+"""The tiled population layout on the host: the index formula the header states, the
+tile/untile converters, buffer lengths (Python against the C entry point) and the argument
+checks of the entry points that take q-major planes only.  No GPU."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+
+def _lat(nx, ny, nz):
+    import tlbm
+    return tlbm.D3Q19(nx, ny, nz)
+
+
+def _params(dims, qstride, tile_rows):
+    from tlbm import _capi
+    p = _capi.Params()
+    p.nx, p.ny, p.nz = dims
+    p.qstride, p.tile_rows = qstride, tile_rows
+    p.obs[0], p.obs[1], p.obs[2] = 2.0, 2.0, 1.0
+    return p
+
+
+@pytest.mark.parametrize("dims,k", [((8, 3, 4), 1), ((8, 3, 4), 4), ((8, 3, 4), 8),
+                                    ((5, 7, 3), 2), ((64, 6, 5), 16)])
+def test_tile_index_formula_and_round_trip(dims, k):
+    """Population q of cell (x, r = z ny + y) sits at (r // k) 19 k nx + q k nx + (r % k) nx + x,
+    checked element by element against an independent loop; untile inverts tile."""
+    import tlbm
+    lat = _lat(*dims)
+    nx, ny, nz = dims
+    rng = np.random.default_rng(7)
+    planes = rng.standard_normal((19, lat.n)).astype(np.float32)
+    flat = tlbm.tile_populations(planes, lat, k)
+    assert flat.size == tlbm.population_elems(lat, None, k)
+    for q in range(19):
+        for z in range(nz):
+            for y in range(ny):
+                r = z * ny + y
+                for x in range(nx):
+                    addr = (r // k) * 19 * k * nx + q * k * nx + (r % k) * nx + x
+                    assert flat[addr] == planes[q, r * nx + x]
+    assert np.array_equal(tlbm.untile_populations(flat, lat, k), planes)
+
+
+def test_population_elems_matches_the_library():
+    import tlbm
+    from tlbm import _capi
+    lib = _capi.lib()
+    for dims, qstride, k in [((8, 3, 4), 96, 0), ((8, 3, 4), 200, 0), ((8, 3, 4), 96, 4),
+                             ((512, 256, 256), None, 4), ((96, 7, 5), 3360, 8),
+                             ((384, 128, 64), None, 32)]:
+        lat = _lat(*dims)
+        p = _params(dims, lat.n if qstride is None else qstride, k)
+        got = ctypes.c_uint64()
+        assert lib.tlbm_population_elems(ctypes.byref(p), ctypes.byref(got)) == 0, \
+            _capi.last_error()
+        assert got.value == tlbm.population_elems(lat, qstride, k)
+    for bad in (3, -1, 1 << 21):
+        p.tile_rows = bad
+        assert lib.tlbm_population_elems(ctypes.byref(p), ctypes.byref(got)) != 0
+        assert "tile_rows" in _capi.last_error()
+
+
+def test_default_tile_rows():
+    """Tiles of at most 2048 cells, only for lattices of 2^20 cells or more with rows of whole
+    waves."""
+    from tlbm import default_tile_rows
+    assert default_tile_rows(512, 256, 256) == 4
+    assert default_tile_rows(384, 256, 256) == 4
+    assert default_tile_rows(256, 64, 64) == 8
+    assert default_tile_rows(1024, 128, 128) == 2
+    assert default_tile_rows(2048, 32, 32) == 1
+    assert default_tile_rows(4096, 32, 32) == 0          # a row beyond one tile's size
+    assert default_tile_rows(480, 256, 256) == 0         # rows not whole waves
+    assert default_tile_rows(128, 64, 64) == 0           # below 2^20 cells
+
+
+def test_q_major_only_entry_points_reject_tiles():
+    """The batch and the adjoint take q-major planes: a tiled params is refused before any
+    allocation or launch."""
+    from tlbm import _capi
+    lib = _capi.lib()
+    p = _params((8, 6, 4), 192, 2)
+    p.tau0 = 0.56
+    h = ctypes.c_int(-1)
+    lib.tlbm_kernel_lookup(ctypes.byref(_capi.Spec(1, 0, 0, 0, 0, 0)), ctypes.byref(h))
+    out = ctypes.c_void_p()
+    assert lib.tlbm_batch_create(h, ctypes.byref(p), 1, ctypes.byref(out)) != 0
+    assert "q-major" in _capi.last_error() and not out.value
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    org = (ctypes.c_int32 * 3)(0, 0, 0)
+    size = (ctypes.c_int32 * 3)(1, 1, 1)
+    p.phi = 0x3000
+    assert lib.tlbm_adjoint_step(ctypes.byref(p), a, b, a, b, 1.0, 1.0, org, size,
+                                 ctypes.c_void_p(0x4000), 0, None) != 0
+    assert "q-major" in _capi.last_error()
