@@ -77,6 +77,10 @@ typedef struct {
                               tile_rows > 0: the temperature planes only)                 */
   int32_t tile_rows;       /* 0: q-major population planes; k > 0 (a power of two): tiles
                               of k rows, 19 planes of k*nx cells each (see the layout note) */
+  int32_t xcd_group;       /* quiet steps: blocks of consecutive cells go to the XCDs in
+                              runs of this many (a power of two up to 64; 1: plain order);
+                              0: 2 for FP16/BF16 storage, 1 for FP32 and the temperature
+                              variant.  Report steps keep plain order.                    */
   float omega;             /* BGK: 1 / (3 nu + 1/2), lbm.py:63                          */
   float tau0;              /* TRT / LES: 3 nu + 1/2, lbm.py:110, 131                    */
   float cs;                /* Smagorinsky constant (TRT.cs, SmagorinskyLES.cs = 0.16)    */

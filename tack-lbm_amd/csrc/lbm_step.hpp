@@ -177,6 +177,8 @@ struct StepArgs {
   uint64_t tile_stride;
   uint32_t tile_log2, tile_mask;
   uint32_t row_waves;        // 1: nx is a multiple of 64, so each wave of 64 cells is in one row
+  uint32_t xcd_group;        // quiet step: runs of this many consecutive blocks per XCD (xcd_chunk)
+  uint32_t nchunks;          // quiet step: blocks of cells (the grid may hold more slots)
   int32_t nx, ny, nz;
   int32_t wrap;              // bit 0/1/2: the obstacle reaches across the x/y/z periodic faces
   uint32_t stream_stores;    // 1: launch the non-temporal-store (NT) quiet kernel (host side)
@@ -910,6 +912,18 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   }
 }
 
+// The chunk (block of consecutive cells) hardware block slot s steps.  Slot s runs on XCD s mod 8
+// (blocks are dispatched to the eight XCDs round-robin); with group g > 1 the slots of one XCD
+// take runs of g consecutive chunks, so the cache lines two neighbouring chunks share (the
+// x-shifted pulls across their common edge) are fetched into one L2 rather than two; g = 1 is
+// plain order.  A grid of a multiple of 8 g slots covers every chunk once; slots whose chunk lies
+// past the last one have nothing to do.
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t s, uint32_t g) {
+  if (g <= 1) return s;
+  const uint32_t x = s & 7u, j = s >> 3;
+  return g * (8u * (j / g) + x) + j % g;
+}
+
 // (x, y, z) of a cell index (x fastest): two divisions by invariant integers, exact for every
 // 32-bit index.
 __device__ __forceinline__ void cell_coords(uint32_t cell, uint32_t nx, uint32_t ny,
@@ -934,7 +948,13 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   static_assert(!HEAT || STORAGE == TLBM_F32, "the temperature variant stores FP32");
   using T = typename StorageT<STORAGE>::T;
   const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * static_cast<uint32_t>(a.nz);
-  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  // report steps keep plain order: their FP64 block partials are indexed by block
+  uint32_t chunk = blockIdx.x;
+  if constexpr (!REPORT) {
+    chunk = xcd_chunk(blockIdx.x, a.xcd_group);
+    if (chunk >= a.nchunks) return;
+  }
+  const uint32_t cell = chunk * blockDim.x + threadIdx.x;
   const bool active = cell < n;
   if constexpr (!REPORT) {
     if (!active) return;
@@ -960,7 +980,9 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
 template <class T, bool NT>
 __global__ void __launch_bounds__(256) pull_copy_kernel(const StepArgs a) {
   const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * static_cast<uint32_t>(a.nz);
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t chunk = xcd_chunk(blockIdx.x, a.xcd_group);
+  if (chunk >= a.nchunks) return;
+  const uint32_t idx = chunk * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   uint32_t x, y, z;
   cell_coords(idx, a.nx, a.ny, a.div_nx, a.div_ny, x, y, z);
@@ -995,29 +1017,19 @@ struct BatchArgs {
   uint32_t nslots;
 };
 
-// The chunk of slot s: with group g > 1, the blocks an XCD runs (slot s goes to XCD s mod 8)
-// take runs of g consecutive chunks, so the cache lines two neighbouring chunks share (their
-// pulls' row and block edges) are fetched into one L2; g = 1 is plain order.  Slots past the
-// last chunk map past it.
-__device__ __forceinline__ uint32_t batch_chunk(uint32_t s, uint32_t g) {
-  if (g <= 1) return s;
-  const uint32_t x = s & 7u, j = s >> 3;
-  return g * (8u * (j / g) + x) + j % g;
-}
-
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool NT = false>
 __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
   using T = typename StorageT<STORAGE>::T;
   const T* __restrict__ fin = static_cast<const T*>(b.fin);
   T* __restrict__ fout = static_cast<T*>(b.fout);
-  // Chunk order (batch_chunk): plain, consecutive chunks on consecutive XCDs, so the eight
+  // Chunk order (xcd_chunk): plain, consecutive chunks on consecutive XCDs, so the eight
   // XCDs stream through nearby addresses of every plane together; or runs of `group`
   // consecutive chunks per XCD, so the lines neighbouring chunks share stay in one L2.
   // Handing each XCD a contiguous eighth of all chunks measured 7-9% slower at FP32 on
   // MI355X: eight streams per plane, each an eighth of the lattice apart, contend for the
   // same HBM channels.
   for (uint32_t slot = blockIdx.x; slot < b.nslots; slot += gridDim.x) {
-    const uint32_t c = batch_chunk(slot, b.group);
+    const uint32_t c = xcd_chunk(slot, b.group);
     if (c >= b.nchunks) continue;  // uniform over the block
     const BatchDomain& D = b.dom[b.chunk_dom[c]];
     const StepArgs& a = D.a;

@@ -30,7 +30,8 @@ class Spec(ctypes.Structure):
 class Params(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
                 ("block_x", ctypes.c_int32), ("qstride", ctypes.c_int64),
-                ("tile_rows", ctypes.c_int32), ("omega", ctypes.c_float), ("tau0", ctypes.c_float), ("cs", ctypes.c_float),
+                ("tile_rows", ctypes.c_int32), ("xcd_group", ctypes.c_int32),
+                ("omega", ctypes.c_float), ("tau0", ctypes.c_float), ("cs", ctypes.c_float),
                 ("magic", ctypes.c_float), ("obs", ctypes.c_float * 8), ("u0", ctypes.c_float),
                 ("sponge_start", ctypes.c_float), ("sponge_strength", ctypes.c_float),
                 ("lid", ctypes.c_float * 3), ("omega_rot", ctypes.c_void_p),

@@ -115,34 +115,44 @@ def tune_batch(batch, reps=5, use_cache=True, candidates=None):
     return best[:3]
 
 
-def tune_solver(solver, reps=5, use_cache=True, blocks=(64, 128, 192, 256)):
-    """Pick threads per block for a Solver's quiet kernel (the report kernel follows it)."""
+def tune_solver(solver, reps=5, use_cache=True, blocks=(64, 128, 192, 256), groups=(1, 2, 4)):
+    """Pick threads per block and the chunk order (``xcd_group``) of a Solver's quiet kernel
+    (the report kernel takes the same block size); applies them with ``solver.set_launch`` and
+    returns (block, ms, xcd_group)."""
     if getattr(solver, "heat_enabled", False):
         raise NotImplementedError("tune_solver times the flow step; this solver has heat")
     L = lib()
     lat = solver.lat
     key = (f"solver|{device_key()}|{solver.collision_name}|{type(solver.geo).__name__}|"
-           f"{type(solver.flow.side).__name__}|{solver.storage}|{lat.nx}x{lat.ny}x{lat.nz}")
+           f"{type(solver.flow.side).__name__}|{solver.storage}|{lat.nx}x{lat.ny}x{lat.nz}|"
+           f"t{solver.tile_rows}")
     table = _load() if use_cache else {}
-    if key in table:
-        block, ms = table[key]
-        solver.set_launch(block)
-        return block, ms
+    if key in table and len(table[key]) == 3:
+        block, ms, group = table[key]
+        solver.set_launch(block, group)
+        return block, ms, group
     from .lbm import make_params
     params = make_params(lat, solver.phys, solver.geo, solver.flow, solver.qstride,
                          solver.tile_rows)
-    best = None
-    for block in blocks:
-        solver.set_launch(block)
-        h = solver._handle(report=False)
+    h = solver._handle(report=False)
+
+    def timed(block, group, n):
+        params.block_x, params.xcd_group = block, group
 
         def launch():
             check(L.tlbm_step(h, solver.f[0].ptr, solver.f[1].ptr, ctypes.byref(params), None,
                               None, solver.stream), "step")
-        ms = _time(launch, solver.stream, reps)
+        return _time(launch, solver.stream, n)
+
+    cands = [(b, g) for b in blocks for g in groups]
+    first = sorted((timed(b, g, reps), i) for i, (b, g) in enumerate(cands))
+    best = None
+    for _, i in first[:4]:  # the four fastest again, timed four times longer
+        b, g = cands[i]
+        ms = timed(b, g, 4 * reps)
         if best is None or ms < best[1]:
-            best = (block, ms)
-    solver.set_launch(best[0])
+            best = (b, ms, g)
+    solver.set_launch(best[0], best[2])
     if use_cache:
         table[key] = list(best)
         _store(table)

@@ -554,17 +554,19 @@ def default_qstride(n):
     return (n + 127) // 128 * 128 + 65792
 
 
-def default_tile_rows(nx, ny, nz):
-    """Rows per population tile for a Solver of nx x ny x nz cells (0: q-major planes).
-    Tiles of about 2048 cells (k = the largest power of two with k nx <= 2048) measured fastest
-    on MI355X for the step's access pattern: 6.05 TB/s against 5.68 for q-major planes of a
-    512x256x256 lattice, 5.9-6.0 for tiles of 1024-16384 cells (no arithmetic, FP32 and FP16
-    alike).  Only for large lattices whose rows are whole waves (nx a multiple of 64), where the
-    16-bit step addresses a wave's source row once for all its lanes."""
-    if nx * ny * nz < (1 << 20) or nx % 64 or nx > 2048:
+def default_tile_rows(nx, ny, nz, storage="f32"):
+    """Rows per population tile for a Solver of nx x ny x nz cells (0: q-major planes): the
+    largest power of two k with k nx cells of one plane in at most 8 KiB (k nx <= 2048 cells at
+    FP32, 4096 at FP16/BF16).  Measured on MI355X at 512x256x256 (LES, quiet step): FP32 0.842 ms
+    with tiles of 4 or 16 rows, 0.851 with 8, 0.859 with 2, against 0.891-0.904 ms for q-major
+    planes; FP16 0.443-0.445 ms with 8 rows, 0.446-0.448 with 4, 0.458-0.461 for planes.  Only
+    for large lattices whose rows are whole waves (nx a multiple of 64), where the 16-bit step
+    addresses a wave's source row once for all its lanes."""
+    per = 8192 // (4 if storage == "f32" else 2)
+    if nx * ny * nz < (1 << 20) or nx % 64 or nx > per:
         return 0
     k = 1
-    while 2 * k * nx <= 2048:
+    while 2 * k * nx <= per:
         k *= 2
     return k
 
@@ -636,7 +638,7 @@ def _fields_struct(fields):
 
 
 def lbm_step(lat, phys, geo, flow, diag, f_in, f_out, heat=None, g_in=None, g_out=None,
-             sums=None, stream=None, block_x=0, tile_rows=0):
+             sums=None, stream=None, block_x=0, tile_rows=0, xcd_group=0):
     """lbm.py:397-513: one step from ``f_in`` to ``f_out``.
 
     ``diag`` is ``Quiet``/``Report``; a ``Report`` step writes its 13 fields and, when ``sums``
@@ -646,7 +648,7 @@ def lbm_step(lat, phys, geo, flow, diag, f_in, f_out, heat=None, g_in=None, g_ou
     ``heat.q_solid``.  ``block_x``: threads per block of this launch (0: the specialisation's
     default).  ``tile_rows`` > 0: the population buffers hold tiles of that many rows
     (``population_elems``); the temperature planes stay q-major, their stride the g buffers'
-    length / 7.
+    length / 7.  ``xcd_group``: a quiet step's chunk order (0: the library's choice).
     """
     if f_in.storage != f_out.storage:
         raise ValueError(f"f_in is {f_in.storage}, f_out is {f_out.storage}")
@@ -665,6 +667,7 @@ def lbm_step(lat, phys, geo, flow, diag, f_in, f_out, heat=None, g_in=None, g_ou
             raise ValueError("f_in and f_out have different plane strides")
     p = make_params(lat, phys, geo, flow, qs, tile_rows)
     p.block_x = int(block_x)
+    p.xcd_group = int(xcd_group)
     fs = _fields_struct(diag.fields) if report else None
     if not hot:
         check(lib().tlbm_step(h, f_in.ptr, f_out.ptr, ctypes.byref(p),
@@ -728,6 +731,7 @@ class Solver:
         self.storage = storage
         self.step_count = 0
         self.block_x = 0
+        self.xcd_group = 0
         self._omega_state = ManagedArray(1)
         side = LidY(*lid_u) if lateral == "lid" else SIDES[lateral]()
         self.flow = Flow(u0, 0.75 * nx if sponge_start is None else sponge_start,
@@ -740,7 +744,8 @@ class Solver:
         self.qstride = default_qstride(n) if qstride is None else int(qstride)
         if self.qstride < n:
             raise ValueError(f"qstride {self.qstride} < cells {n}")
-        self.tile_rows = default_tile_rows(nx, ny, nz) if tile_rows is None else int(tile_rows)
+        self.tile_rows = (default_tile_rows(nx, ny, nz, storage) if tile_rows is None
+                          else int(tile_rows))
         if self.tile_rows < 0 or self.tile_rows & (self.tile_rows - 1):
             raise ValueError(f"tile_rows {self.tile_rows} is not 0 or a power of two")
         elems = population_elems(self.lat, self.qstride, self.tile_rows)
@@ -839,13 +844,18 @@ class Solver:
             self.update_output()
         sync(self.stream)
 
-    def set_launch(self, block_x=0):
-        """Threads per block of this solver's steps (64, 128, 192 or 256; 0: the default).  The
-        choice belongs to this solver only (it travels in its step parameters)."""
+    def set_launch(self, block_x=0, xcd_group=0):
+        """Threads per block of this solver's steps (64, 128, 192 or 256; 0: the default) and
+        the quiet steps' chunk order (blocks go to the XCDs in runs of ``xcd_group``, a power of
+        two up to 64; 0: the library's choice for the storage).  The choice belongs to this
+        solver only (it travels in its step parameters)."""
         if block_x not in (0, 64, 128, 192, 256):
             raise ValueError(f"block_x must be 0, 64, 128, 192 or 256, got {block_x}")
+        if xcd_group < 0 or xcd_group > 64 or xcd_group & (xcd_group - 1):
+            raise ValueError(f"xcd_group must be 0 or a power of two up to 64, got {xcd_group}")
         self._drop_graphs()
         self.block_x = int(block_x)
+        self.xcd_group = int(xcd_group)
 
     def set_physics(self, collision, reynolds):
         """lbm.py:613-616: nu = u0 D / Re; a collision change selects another kernel."""
@@ -906,7 +916,8 @@ class Solver:
         lbm_step(self.lat, self.phys, self.geo, self.flow, diag, self.f[0], self.f[1],
                  self.heat, self.g[0], self.g[1], sums=self.sums if report else None,
                  stream=self.stream, block_x=getattr(self, "block_x", 0),
-                 tile_rows=getattr(self, "tile_rows", 0))
+                 tile_rows=getattr(self, "tile_rows", 0),
+                 xcd_group=getattr(self, "xcd_group", 0))
         self.f.reverse()
         self.g.reverse()
         self.step_count += 1

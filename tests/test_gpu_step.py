@@ -396,6 +396,30 @@ def test_capture_refuses_what_a_graph_must_not_hold():
     L.tlbm_stream_destroy(st)
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_quiet_chunk_orders_step_as_plain_order(storage):
+    """Every chunk order (blocks to XCDs in runs of 1..64, grids rounded up to whole runs, the
+    extra slots idle) and block size steps every cell exactly as plain order does; 750 blocks
+    of 256 cells is not a multiple of any run of 8 groups."""
+    import tlbm
+    kw = dict(collision="les", obstacle="sphere", reynolds=400.0, diameter=10.0,
+              noise_amp=1e-3, storage=storage)
+    ref = tlbm.Solver(200, 40, 24, **kw)
+    ref.set_launch(256, 1)
+    for _ in range(3):
+        ref.step()
+    want = ref.populations()
+    for block, group in ((256, 2), (256, 4), (256, 64), (64, 8), (128, 2), (192, 16), (0, 0)):
+        print(f"[order] {storage} block {block} group {group}", flush=True)
+        s = tlbm.Solver(200, 40, 24, **kw)
+        s.set_launch(block, group)
+        for _ in range(3):
+            s.step()
+        assert np.array_equal(s.populations(), want), (block, group)
+    with pytest.raises(ValueError):
+        s.set_launch(256, 3)
+
+
 def test_solver_autotune_keeps_state_and_caches(tmp_path, monkeypatch):
     """tune_solver times each block size on the solver's own buffers without changing its
     state, applies the fastest, and answers the second call from its cache; it refuses a
@@ -406,10 +430,11 @@ def test_solver_autotune_keeps_state_and_caches(tmp_path, monkeypatch):
     s = tlbm.Solver(200, 40, 24, collision="les", obstacle="sphere", reynolds=400.0,
                     diameter=10.0, noise_amp=1e-3)
     before = s.populations()
-    block, ms = autotune.tune_solver(s, reps=2)
-    assert block in (64, 128, 192, 256) and ms > 0
+    block, ms, group = autotune.tune_solver(s, reps=2)
+    assert block in (64, 128, 192, 256) and group in (1, 2, 4) and ms > 0
+    assert (s.block_x, s.xcd_group) == (block, group)
     assert np.array_equal(s.populations(), before)
-    assert autotune.tune_solver(s, reps=2)[0] == block
+    assert autotune.tune_solver(s, reps=2)[::2] == (block, group)
     s.step()                                  # the tuned launch steps as the default does
     ref = tlbm.Solver(200, 40, 24, collision="les", obstacle="sphere", reynolds=400.0,
                       diameter=10.0, noise_amp=1e-3)

@@ -77,8 +77,8 @@ def test_default_tiles_on_a_large_lattice(storage, dims):
     """A lattice large enough for the default tiles and the non-temporal-store kernels."""
     import tlbm
     print(f"[tiles] default tiles {storage} {dims}", flush=True)
-    k = tlbm.default_tile_rows(*dims)
-    assert k == 8
+    k = tlbm.default_tile_rows(*dims, storage)
+    assert k == (8 if storage == "f32" else 16)
     a, b = _pair("les", "sphere", "lid", dims, k, storage)
     for s in (a, b):
         for i in range(4):

@@ -61,18 +61,26 @@ def test_population_elems_matches_the_library():
         p.tile_rows = bad
         assert lib.tlbm_population_elems(ctypes.byref(p), ctypes.byref(got)) != 0
         assert "tile_rows" in _capi.last_error()
+    p.tile_rows = 4
+    for bad in (3, -2, 128):
+        p.xcd_group = bad
+        assert lib.tlbm_population_elems(ctypes.byref(p), ctypes.byref(got)) != 0
+        assert "xcd_group" in _capi.last_error()
 
 
 def test_default_tile_rows():
-    """Tiles of at most 2048 cells, only for lattices of 2^20 cells or more with rows of whole
-    waves."""
+    """Tiles whose plane segment holds at most 8 KiB, only for lattices of 2^20 cells or more
+    with rows of whole waves."""
     from tlbm import default_tile_rows
     assert default_tile_rows(512, 256, 256) == 4
+    assert default_tile_rows(512, 256, 256, "f16") == 8
+    assert default_tile_rows(512, 256, 256, "bf16") == 8
     assert default_tile_rows(384, 256, 256) == 4
     assert default_tile_rows(256, 64, 64) == 8
     assert default_tile_rows(1024, 128, 128) == 2
     assert default_tile_rows(2048, 32, 32) == 1
-    assert default_tile_rows(4096, 32, 32) == 0          # a row beyond one tile's size
+    assert default_tile_rows(4096, 32, 32) == 0          # a row beyond 8 KiB at FP32
+    assert default_tile_rows(4096, 32, 32, "f16") == 1
     assert default_tile_rows(480, 256, 256) == 0         # rows not whole waves
     assert default_tile_rows(128, 64, 64) == 0           # below 2^20 cells
 
