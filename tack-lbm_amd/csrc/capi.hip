@@ -915,12 +915,19 @@ int tlbm_pull_copy(const void* f_in, void* f_out, const tlbm_params* params, int
   if (slots * 256 > 0xFFFFFFFFull) return fail(TLBM_ERR_INVALID, "grid exceeds the launch limit");
   const dim3 grid(static_cast<uint32_t>(slots)), block(256);
   hipStream_t st = static_cast<hipStream_t>(stream);
+  // the quiet step's builds (registry.hpp launch_step): non-temporal stores on large lattices,
+  // per-wave row addressing where rows are whole waves
+  const bool rows = a.row_waves != 0;
   if (storage == TLBM_F32) {
-    if (nt) hipLaunchKernelGGL((tlbm::pull_copy_kernel<float, true>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((tlbm::pull_copy_kernel<float, false>), grid, block, 0, st, a);
+    if (nt && rows) hipLaunchKernelGGL((tlbm::pull_copy_kernel<float, true, true>), grid, block, 0, st, a);
+    else if (nt) hipLaunchKernelGGL((tlbm::pull_copy_kernel<float, true, false>), grid, block, 0, st, a);
+    else if (rows) hipLaunchKernelGGL((tlbm::pull_copy_kernel<float, false, true>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((tlbm::pull_copy_kernel<float, false, false>), grid, block, 0, st, a);
   } else {
-    if (nt) hipLaunchKernelGGL((tlbm::pull_copy_kernel<uint16_t, true>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((tlbm::pull_copy_kernel<uint16_t, false>), grid, block, 0, st, a);
+    if (nt && rows) hipLaunchKernelGGL((tlbm::pull_copy_kernel<uint16_t, true, true>), grid, block, 0, st, a);
+    else if (nt) hipLaunchKernelGGL((tlbm::pull_copy_kernel<uint16_t, true, false>), grid, block, 0, st, a);
+    else if (rows) hipLaunchKernelGGL((tlbm::pull_copy_kernel<uint16_t, false, true>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((tlbm::pull_copy_kernel<uint16_t, false, false>), grid, block, 0, st, a);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "pull copy launch");

@@ -118,14 +118,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
 }
 
-template <class T>
+// NT: the non-temporal cache policy (the "nt" bit, 2 in the cache-policy operand), for lines
+// read once
+template <class T, bool NT = false>
 __device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  constexpr int pol = NT ? 2 : 0;
   if constexpr (sizeof(T) == 4) {
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(
-                                     r, static_cast<int>(voff), static_cast<int>(soff), 0));
+                                     r, static_cast<int>(voff), static_cast<int>(soff), pol));
   } else {
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(
-                                     r, static_cast<int>(voff), static_cast<int>(soff), 0));
+                                     r, static_cast<int>(voff), static_cast<int>(soff), pol));
   }
 }
 
@@ -532,52 +535,130 @@ __device__ __forceinline__ void collide_cell(const StepArgs& a, float (&v)[kQ], 
   uz_out = uz;
 }
 
-// ---- XCD-aware block -> tile mapping -----------------------------------------------------------
 // ---- the 19 pull loads of one cell (lbm.py:410-426), raw stored values ------------------------
 // Sources: x and z periodic (lbm.py:412, 415), y by side (lbm.py:342-369): periodic, or the
 // y-mirrored population of the same row at a slip face; ceiling links of the lid read their
 // in-bounds (clamped) row and are overwritten by the caller.
-template <int SIDE, class T>
+// ROWS: rows are whole waves (nx a multiple of 64; blocks of consecutive cells then put every
+// wave in one row), so y and z are the same for all lanes of a wave, and so are the source
+// row, plane and y mirror of every population, on the faces too.  The wave forms each source
+// row's address in scalar registers and the lanes add only their source column (x, or x -+ 1
+// with the row's periodic wrap): 16-bit storage through one buffer resource (plane and row as
+// the scalar offset), FP32 as global loads from a scalar base.  A compile-time choice (the host
+// launches the ROWS build where nx % 64 == 0): with a runtime branch the compiler merged the
+// two load sequences into one with per-lane 64-bit addresses.
+// NTL (with ROWS): the nine populations with c_x = 0 are loaded with the non-temporal policy.
+// Their lines are read by this wave alone, once per step, while the lines of the ten x-shifted
+// loads are shared with the neighbouring wave (the +-1 cell at each wave's edge) and keep the
+// default policy.  Measured on MI355X at 512x256x256 (LES, sphere, lid, tiles of rows,
+// non-temporal stores), in one process against plain loads: FP32 0.834 -> 0.824 ms, FP16
+// 0.409-0.413 -> 0.407-0.408 ms (all 19 loads non-temporal: slower than plain).
+template <bool NT, class T>
+__device__ __forceinline__ T pull_load(const T* __restrict__ p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+template <int SIDE, bool ROWS = false, bool NTL = false, class T>
 __device__ __forceinline__ void pull_raw(const StepArgs& a, const T* __restrict__ fin, int x, int y,
                                          int z, T (&raw)[kQ]) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const uint32_t snx = static_cast<uint32_t>(nx);
-  const uint32_t row0 = static_cast<uint32_t>(z) * ny + y;
-  const uint32_t idx = row0 * snx + x;
   const int xsrc[3] = {x == nx - 1 ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};
-  const int zsrc[3] = {z == nz - 1 ? 0 : z + 1, z, z == 0 ? nz - 1 : z - 1};
-  int ysrc[3];
-  bool mir_lo = false, mir_hi = false;
-  if constexpr (SIDE == TLBM_PERIODIC_Y) {
-    ysrc[0] = y == ny - 1 ? 0 : y + 1;
-    ysrc[2] = y == 0 ? ny - 1 : y - 1;
-  } else {
-    ysrc[0] = y == ny - 1 ? y : y + 1;
-    ysrc[2] = y == 0 ? y : y - 1;
-    mir_lo = (y == 0);
-    if constexpr (SIDE == TLBM_SLIP_Y) mir_hi = (y == ny - 1);
-  }
-  ysrc[1] = y;
   // No branch between the 19 loads of a lane, so that all of them are in flight together.
-  // 16-bit storage, interior waves: when no lane of the wave lies on a y or z face, the source
-  // of population q is idx - (cx + cy nx + cz nx ny) for every lane, except the x wrap of one
-  // lane at each row end.  So the 19 loads go through one buffer resource with the plane
-  // offsets in scalar registers (precomputed on the host) and three shared 32-bit lane offsets:
-  // a handful of vector instructions instead of a 64-bit address per load and lane (472 -> 347
-  // VALU instructions per wave of the FP16 LES step, whose VALU was ~100% busy on a box where
-  // it took 0.46 ms).  Waves that touch a y or z face (mirror, lid or wrap per lane) take the
-  // general per-lane path.  FP32 keeps the general path everywhere: it is bandwidth-bound, and
-  // its buffer loads measured 2% slower than per-lane global loads (0.912 against 0.893 ms at
-  // 512x256x256 in one process, same box).
-  // Tiles of rows: when each wave lies in one row (row_waves), the source row of population q
-  // is the same for the whole wave, so its offset (tile, plane, row in the tile) is computed in
-  // scalar registers and the lanes add only their source column.
-  const bool inner = y >= 1 && y < ny - 1 && z >= 1 && z < nz - 1;
-  if (sizeof(T) == 2 && a.off32 && (a.tile_stride == 0 || a.row_waves) &&
-      __builtin_amdgcn_ballot_w64(!inner) == 0) {
-    constexpr uint32_t B = sizeof(T);
-    const auto rin = plane_rsrc(fin);
-    if (a.tile_stride == 0) {
+  if constexpr (ROWS) {
+    const uint32_t yu = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(y));
+    const uint32_t zu = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(z));
+    const uint32_t uny = static_cast<uint32_t>(ny), unz = static_cast<uint32_t>(nz);
+    const uint32_t zs[3] = {zu == unz - 1 ? 0u : zu + 1, zu, zu == 0 ? unz - 1 : zu - 1};
+    uint32_t ys[3];
+    bool lo = false, hi = false;
+    if constexpr (SIDE == TLBM_PERIODIC_Y) {
+      ys[0] = yu == uny - 1 ? 0u : yu + 1;
+      ys[2] = yu == 0 ? uny - 1 : yu - 1;
+    } else {
+      ys[0] = yu == uny - 1 ? yu : yu + 1;
+      ys[2] = yu == 0 ? yu : yu - 1;
+      lo = (yu == 0);
+      if constexpr (SIDE == TLBM_SLIP_Y) hi = (yu == uny - 1);
+    }
+    ys[1] = yu;
+    const uint32_t col[3] = {static_cast<uint32_t>(xsrc[0]) * static_cast<uint32_t>(sizeof(T)),
+                             static_cast<uint32_t>(x) * static_cast<uint32_t>(sizeof(T)),
+                             static_cast<uint32_t>(xsrc[2]) * static_cast<uint32_t>(sizeof(T))};
+    // the off32 test stays outside the loads: a branch between them would hold back the later
+    // ones (and the compiler does not hoist it out of the unrolled sequence)
+    if (sizeof(T) == 2 && a.off32) {
+      // the whole buffer within 2^32 bytes: 32-bit row offsets
+      const auto rin = plane_rsrc(fin);
+      const uint32_t ts = static_cast<uint32_t>(a.tile_stride);
+      if (yu >= 1 && yu < uny - 1 && zu >= 1 && zu < unz - 1) {
+        // a wave on no y or z face (98% of them at 512x256x256): source row r0 - (c_y + c_z ny),
+        // no wrap or mirror -- the fewest scalar instructions, which the VALU-heavy 16-bit step
+        // measured (1% faster than the general form below)
+        const uint32_t r0 = zu * uny + yu;
+        static_for<0, kQ>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          constexpr int cx = kCX[q];
+          const uint32_t r = r0 - static_cast<uint32_t>(kCY[q] + kCZ[q] * ny);
+          const uint32_t so = ((r >> a.tile_log2) * ts + (r & a.tile_mask) * snx) *
+                                  static_cast<uint32_t>(sizeof(T)) + a.plane_off[q];
+          raw[q] = buffer_load<T, NTL && cx == 0>(rin, col[1 + cx], so);
+        });
+        return;
+      }
+      static_for<0, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+        int qs = q;
+        if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = lo ? miry(q) : q;
+        if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = hi ? miry(q) : q;
+        const uint32_t r = zs[1 + cz] * uny + ys[1 + cy];
+        const uint32_t so = ((r >> a.tile_log2) * ts + (r & a.tile_mask) * snx) *
+                                static_cast<uint32_t>(sizeof(T)) + a.plane_off[qs];
+        raw[q] = buffer_load<T, NTL && cx == 0>(rin, col[1 + cx], so);
+      });
+    } else {
+      static_for<0, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+        int qs = q;
+        if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = lo ? miry(q) : q;
+        if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = hi ? miry(q) : q;
+        const uint64_t e = static_cast<uint64_t>(qs) * a.ns + row_elem(a, zs[1 + cz] * uny + ys[1 + cy]);
+        // a scalar base plus a 32-bit lane offset: the global_load's saddr form
+        raw[q] = pull_load<NTL && cx == 0>(
+            reinterpret_cast<const T*>(reinterpret_cast<const char*>(fin + e) + col[1 + cx]));
+      });
+    }
+  } else {
+    const uint32_t idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
+    const int zsrc[3] = {z == nz - 1 ? 0 : z + 1, z, z == 0 ? nz - 1 : z - 1};
+    int ysrc[3];
+    bool mir_lo = false, mir_hi = false;
+    if constexpr (SIDE == TLBM_PERIODIC_Y) {
+      ysrc[0] = y == ny - 1 ? 0 : y + 1;
+      ysrc[2] = y == 0 ? ny - 1 : y - 1;
+    } else {
+      ysrc[0] = y == ny - 1 ? y : y + 1;
+      ysrc[2] = y == 0 ? y : y - 1;
+      mir_lo = (y == 0);
+      if constexpr (SIDE == TLBM_SLIP_Y) mir_hi = (y == ny - 1);
+    }
+    ysrc[1] = y;
+    // Other rows, 16-bit storage, q-major planes, interior waves (no lane on a y or z face): the
+    // source of population q is idx - (cx + cy nx + cz nx ny) for every lane, except the x wrap
+    // of one lane at each row end, so the loads go through one buffer resource with the plane
+    // offsets less the pull shift precomputed on the host (pull_off) and three shared lane
+    // offsets.  The rest (FP32, face waves, tiles, buffers beyond 2^32 bytes) forms a 64-bit
+    // address per load and lane.  Measured on the FP16 LES step before the buffer paths: 472
+    // VALU instructions per wave against 347, on a box where the step's VALU was ~100% busy
+    // (0.46 ms at 512x256x256).
+    const bool inner = y >= 1 && y < ny - 1 && z >= 1 && z < nz - 1;
+    if (sizeof(T) == 2 && a.off32 && a.tile_stride == 0 &&
+        __builtin_amdgcn_ballot_w64(!inner) == 0) {
+      constexpr uint32_t B = sizeof(T);
+      const auto rin = plane_rsrc(fin);
       const uint32_t v0 = idx * B;
       const uint32_t vp = (idx + (x == 0 ? snx : 0u)) * B;       // c_x = +1: x - 1, wrapped at 0
       const uint32_t vm = (idx - (x == nx - 1 ? snx : 0u)) * B;  // c_x = -1: x + 1, wrapped
@@ -587,31 +668,18 @@ __device__ __forceinline__ void pull_raw(const StepArgs& a, const T* __restrict_
         raw[q] = buffer_load<T>(rin, cx == 1 ? vp : (cx == -1 ? vm : v0), a.pull_off[q]);
       });
     } else {
-      const uint32_t r0 = __builtin_amdgcn_readfirstlane(row0);
-      const uint32_t ts = static_cast<uint32_t>(a.tile_stride);
-      const uint32_t v0 = static_cast<uint32_t>(x) * B;
-      const uint32_t vp = static_cast<uint32_t>(xsrc[2]) * B;  // c_x = +1
-      const uint32_t vm = static_cast<uint32_t>(xsrc[0]) * B;  // c_x = -1
       static_for<0, kQ>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int cx = kCX[q];
-        const uint32_t r = r0 - static_cast<uint32_t>(kCY[q] + kCZ[q] * ny);  // no face: no wrap
-        const uint32_t so = ((r >> a.tile_log2) * ts + (r & a.tile_mask) * snx) * B + a.plane_off[q];
-        raw[q] = buffer_load<T>(rin, cx == 1 ? vp : (cx == -1 ? vm : v0), so);
+        constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
+        int qs = q;
+        if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = mir_lo ? miry(q) : q;
+        if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = mir_hi ? miry(q) : q;
+        const T* __restrict__ row =
+            fin + static_cast<uint64_t>(qs) * a.ns +
+            row_elem(a, static_cast<uint32_t>(zsrc[1 + cz]) * ny + static_cast<uint32_t>(ysrc[1 + cy]));
+        raw[q] = row[xsrc[1 + cx]];
       });
     }
-  } else {
-    static_for<0, kQ>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      constexpr int cx = kCX[q], cy = kCY[q], cz = kCZ[q];
-      int qs = q;
-      if constexpr (SIDE != TLBM_PERIODIC_Y && cy == 1) qs = mir_lo ? miry(q) : q;
-      if constexpr (SIDE == TLBM_SLIP_Y && cy == -1) qs = mir_hi ? miry(q) : q;
-      const T* __restrict__ row =
-          fin + static_cast<uint64_t>(qs) * a.ns +
-          row_elem(a, static_cast<uint32_t>(zsrc[1 + cz]) * ny + static_cast<uint32_t>(ysrc[1 + cy]));
-      raw[q] = row[xsrc[1 + cx]];
-    });
   }
 }
 
@@ -619,20 +687,42 @@ __device__ __forceinline__ void pull_raw(const StepArgs& a, const T* __restrict_
 // NT: non-temporal stores, for lattices whose two population buffers are far larger than the
 // Infinity Cache (measured on MI355X: 1.6% faster at FP32, 3.4% at FP16, 512x256x256); below
 // that size the next step's loads hit the cache lines plain stores leave there.
-// own: the cell's element in plane 0 (row_elem of its row + x).
-template <bool NT, class T>
-__device__ __forceinline__ void store_raw(const StepArgs& a, T* __restrict__ fout, uint64_t own,
-                                          const T (&raw)[kQ]) {
-  if (sizeof(T) == 2 && a.off32) {
+// row: the first element of the cell's row in plane 0 (row_elem), x its column.  ROWS (rows of
+// whole waves, see pull_raw): the row is the same for the whole wave and its address is formed
+// in scalar registers.
+template <bool NT, bool ROWS = false, class T>
+__device__ __forceinline__ void store_raw(const StepArgs& a, T* __restrict__ fout, uint64_t row,
+                                          uint32_t x, const T (&raw)[kQ]) {
+  constexpr uint32_t B = sizeof(T);
+  if constexpr (ROWS) {
+    const uint64_t ru = (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(
+                             static_cast<uint32_t>(row >> 32))) << 32) |
+                        __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(row));
+    if (sizeof(T) == 2 && a.off32) {
+      const auto rout = plane_rsrc(fout);
+      static_for<0, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        buffer_store<T, NT>(raw[q], rout, x * B, static_cast<uint32_t>(ru) * B + a.plane_off[q]);
+      });
+    } else {
+      static_for<0, kQ>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        T* const dst = reinterpret_cast<T*>(
+            reinterpret_cast<char*>(fout + (static_cast<uint64_t>(q) * a.ns + ru)) + x * B);
+        if constexpr (NT) __builtin_nontemporal_store(raw[q], dst);
+        else *dst = raw[q];
+      });
+    }
+  } else if (sizeof(T) == 2 && a.off32) {
     // one lane offset for all 19 planes, the plane offsets in scalar registers
-    constexpr uint32_t B = sizeof(T);
     const auto rout = plane_rsrc(fout);
-    const uint32_t vo = static_cast<uint32_t>(own) * B;
+    const uint32_t vo = static_cast<uint32_t>(row + x) * B;
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       buffer_store<T, NT>(raw[q], rout, vo, a.plane_off[q]);
     });
   } else {
+    const uint64_t own = row + x;
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       T* const dst = (fout + static_cast<uint64_t>(q) * a.ns) + own;
@@ -737,8 +827,10 @@ __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z
   }
 }
 
+// NT: non-temporal population stores (and, with ROWS, non-temporal loads of the unshifted
+// pulls); ROWS: rows of whole waves, addressed per wave (pull_raw).
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool HEAT = false,
-          bool NT = false>
+          bool NT = false, bool ROWS = false>
 __device__ __forceinline__ void step_cell(const StepArgs& a, const typename StorageT<STORAGE>::T* __restrict__ fin,
                                           typename StorageT<STORAGE>::T* __restrict__ fout,
                                           int x, int y, int z, bool active,
@@ -751,8 +843,9 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const uint32_t snx = static_cast<uint32_t>(nx);
   const uint32_t idx = (static_cast<uint32_t>(z) * ny + y) * snx + x;
-  // the cell's own element in population plane 0 (idx for q-major planes)
-  const uint64_t own_e = row_elem(a, static_cast<uint32_t>(z) * ny + y) + static_cast<uint32_t>(x);
+  // the cell's row and own element in population plane 0 (idx for q-major planes)
+  const uint64_t own_row = row_elem(a, static_cast<uint32_t>(z) * ny + y);
+  const uint64_t own_e = own_row + static_cast<uint32_t>(x);
 
   // pull sources (x, z periodic: lbm.py:412, 415; y by side: lbm.py:342-369), indexed [1 + c]:
   // the source of a population moving with +c is one cell back.
@@ -778,7 +871,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   float f[kQ];
   {
     T raw[kQ];
-    pull_raw<SIDE>(a, fin, x, y, z, raw);
+    pull_raw<SIDE, ROWS, NT && ROWS>(a, fin, x, y, z, raw);
     static_for<0, kQ>([&](auto qc) { f[decltype(qc)::value] = St::value(raw[decltype(qc)::value]); });
   }
 
@@ -868,7 +961,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const typename Stor
   if (active) {
     T raw[kQ];
     static_for<0, kQ>([&](auto qc) { raw[decltype(qc)::value] = St::encode(f[decltype(qc)::value]); });
-    store_raw<NT>(a, fout, own_e, raw);
+    store_raw<NT, ROWS>(a, fout, own_row, static_cast<uint32_t>(x), raw);
   }
   if constexpr (HEAT) heat_cell<OBS, REPORT, NT>(a, x, y, z, idx, here, ux, uy, uz, ex, active);
 
@@ -943,7 +1036,7 @@ __device__ __forceinline__ void cell_coords(uint32_t cell, uint32_t nx, uint32_t
 // up to 4% faster on rows that are a multiple of 64 cells, and 1.5-2x faster on rows that are
 // not (their waves started off a cache line, and two XCDs wrote the line a row ends in).
 template <int COLL, int OBS, int SIDE, bool REPORT, int STORAGE, bool HEAT = false,
-          bool NT = false>
+          bool NT = false, bool ROWS = false>
 __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   static_assert(!HEAT || STORAGE == TLBM_F32, "the temperature variant stores FP32");
   using T = typename StorageT<STORAGE>::T;
@@ -965,7 +1058,7 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   uint32_t x, y, z;
   cell_coords(idx, a.nx, a.ny, a.div_nx, a.div_ny, x, y, z);
   double acc[TLBM_NSUMS];
-  step_cell<COLL, OBS, SIDE, REPORT, STORAGE, HEAT, NT>(
+  step_cell<COLL, OBS, SIDE, REPORT, STORAGE, HEAT, NT, ROWS>(
       a, static_cast<const T*>(a.fin), static_cast<T*>(a.fout), static_cast<int>(x),
       static_cast<int>(y), static_cast<int>(z), active, a.fld, idx, acc);
   if constexpr (REPORT) {
@@ -977,7 +1070,7 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
 // f_out[q][idx] = f_in[q][source of q] for the 19 populations, the sources of a periodic lattice,
 // with the step's mapping, index split, loads and stores and no arithmetic: the bandwidth this
 // access pattern reaches is the ceiling of the fused step.  T: the stored element (4 or 2 bytes).
-template <class T, bool NT>
+template <class T, bool NT, bool ROWS = false>
 __global__ void __launch_bounds__(256) pull_copy_kernel(const StepArgs a) {
   const uint32_t n = static_cast<uint32_t>(a.nx) * a.ny * static_cast<uint32_t>(a.nz);
   const uint32_t chunk = xcd_chunk(blockIdx.x, a.xcd_group);
@@ -987,9 +1080,9 @@ __global__ void __launch_bounds__(256) pull_copy_kernel(const StepArgs a) {
   uint32_t x, y, z;
   cell_coords(idx, a.nx, a.ny, a.div_nx, a.div_ny, x, y, z);
   T raw[kQ];
-  pull_raw<TLBM_PERIODIC_Y>(a, static_cast<const T*>(a.fin), static_cast<int>(x),
+  pull_raw<TLBM_PERIODIC_Y, ROWS, NT && ROWS>(a, static_cast<const T*>(a.fin), static_cast<int>(x),
                             static_cast<int>(y), static_cast<int>(z), raw);
-  store_raw<NT>(a, static_cast<T*>(a.fout), row_elem(a, z * a.ny + y) + x, raw);
+  store_raw<NT, ROWS>(a, static_cast<T*>(a.fout), row_elem(a, z * a.ny + y), x, raw);
 }
 
 // ---- batched lattices: many independent domains in one persistent launch -------------------------
@@ -1041,7 +1134,9 @@ __global__ void __launch_bounds__(256) lbm_batch_kernel(const BatchArgs b) {
     uint32_t x, y, z;
     cell_coords(cell, a.nx, a.ny, a.div_nx, a.div_ny, x, y, z);
     double acc[TLBM_NSUMS];
-    step_cell<COLL, OBS, SIDE, REPORT, STORAGE, false, NT>(a, fin + D.elem_off, fout + D.elem_off,
+    // per-lane addressing and plain loads (ROWS false): the domains' rows need not be whole
+    // waves (non-temporal unshifted loads measured 4% slower on the FP32 sweep)
+    step_cell<COLL, OBS, SIDE, REPORT, STORAGE, false, NT, false>(a, fin + D.elem_off, fout + D.elem_off,
                                                 static_cast<int>(x), static_cast<int>(y),
                                                 static_cast<int>(z), active, b.fld,
                                                 D.cell_off + cell, acc);

@@ -15,18 +15,25 @@ using StepTable = StepLaunch[kObstacles][3][2];
 // Launchers of the temperature variant of one collision (FP32), indexed as StepTable.
 using HeatTable = StepLaunch[kObstacles][3][2];
 
-// Quiet steps come in two builds: plain and non-temporal population stores (NT), chosen per
-// launch by a.stream_stores (the host sets it for large lattices).  Report steps always use
-// plain stores.
+// Quiet steps come in four builds: plain or non-temporal population stores (NT), chosen per
+// launch by a.stream_stores (the host sets it for large lattices), each with per-lane or
+// per-wave row addressing (ROWS, for rows that are whole waves: a.row_waves); the NT + ROWS
+// build also streams the unshifted pulls with non-temporal loads.  Report steps use plain
+// loads and stores.
 template <int C, int O, int S, bool R, int ST, bool H = false>
 hipError_t launch_step(const StepArgs& a, dim3 grid, dim3 block, hipStream_t stream) {
   if constexpr (!R) {
-    if (a.stream_stores) {
-      hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, true>), grid, block, 0, stream, a);
-      return hipGetLastError();
-    }
+    if (a.stream_stores && a.row_waves)
+      hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, true, true>), grid, block, 0, stream, a);
+    else if (a.stream_stores)
+      hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, true, false>), grid, block, 0, stream, a);
+    else if (a.row_waves)
+      hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, false, true>), grid, block, 0, stream, a);
+    else
+      hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, false, false>), grid, block, 0, stream, a);
+    return hipGetLastError();
   }
-  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, false>), grid, block, 0, stream, a);
+  hipLaunchKernelGGL((lbm_step_kernel<C, O, S, R, ST, H, false, false>), grid, block, 0, stream, a);
   return hipGetLastError();
 }
 
