@@ -688,33 +688,27 @@ __device__ __forceinline__ void pull_raw(const StepArgs& a, const T* __restrict_
 // Infinity Cache (measured on MI355X: 1.6% faster at FP32, 3.4% at FP16, 512x256x256); below
 // that size the next step's loads hit the cache lines plain stores leave there.
 // row: the first element of the cell's row in plane 0 (row_elem), x its column.  ROWS (rows of
-// whole waves, see pull_raw): the row is the same for the whole wave and its address is formed
-// in scalar registers.
+// whole waves, see pull_raw), FP32: the row is the same for the whole wave and its address is
+// formed in scalar registers (global stores from a scalar base: 0.832 -> 0.828 ms at
+// 512x256x256).  16-bit storage keeps one lane offset for all 19 planes and the plane offsets
+// as the scalar operand (with the row in the scalar offset too, the FP16 step measured 1.3%
+// slower).
 template <bool NT, bool ROWS = false, class T>
 __device__ __forceinline__ void store_raw(const StepArgs& a, T* __restrict__ fout, uint64_t row,
                                           uint32_t x, const T (&raw)[kQ]) {
   constexpr uint32_t B = sizeof(T);
-  if constexpr (ROWS) {
+  if constexpr (ROWS && sizeof(T) == 4) {
     const uint64_t ru = (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(
                              static_cast<uint32_t>(row >> 32))) << 32) |
                         __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(row));
-    if (sizeof(T) == 2 && a.off32) {
-      const auto rout = plane_rsrc(fout);
-      static_for<0, kQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        buffer_store<T, NT>(raw[q], rout, x * B, static_cast<uint32_t>(ru) * B + a.plane_off[q]);
-      });
-    } else {
-      static_for<0, kQ>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        T* const dst = reinterpret_cast<T*>(
-            reinterpret_cast<char*>(fout + (static_cast<uint64_t>(q) * a.ns + ru)) + x * B);
-        if constexpr (NT) __builtin_nontemporal_store(raw[q], dst);
-        else *dst = raw[q];
-      });
-    }
+    static_for<0, kQ>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      T* const dst = reinterpret_cast<T*>(
+          reinterpret_cast<char*>(fout + (static_cast<uint64_t>(q) * a.ns + ru)) + x * B);
+      if constexpr (NT) __builtin_nontemporal_store(raw[q], dst);
+      else *dst = raw[q];
+    });
   } else if (sizeof(T) == 2 && a.off32) {
-    // one lane offset for all 19 planes, the plane offsets in scalar registers
     const auto rout = plane_rsrc(fout);
     const uint32_t vo = static_cast<uint32_t>(row + x) * B;
     static_for<0, kQ>([&](auto qc) {
@@ -1056,7 +1050,16 @@ __global__ void __launch_bounds__(256) lbm_step_kernel(const StepArgs a) {
   // mirror it (in-bounds loads) and store nothing
   const uint32_t idx = active ? cell : n - 1;
   uint32_t x, y, z;
-  cell_coords(idx, a.nx, a.ny, a.div_nx, a.div_ny, x, y, z);
+  if constexpr (ROWS) {
+    // rows of whole waves: the wave's row, y and z in scalar registers (the face, lid and row
+    // addressing logic then runs on the scalar unit), only x per lane
+    const uint32_t row = divide(__builtin_amdgcn_readfirstlane(idx), a.div_nx);
+    x = idx - row * static_cast<uint32_t>(a.nx);
+    z = divide(row, a.div_ny);
+    y = row - z * static_cast<uint32_t>(a.ny);
+  } else {
+    cell_coords(idx, a.nx, a.ny, a.div_nx, a.div_ny, x, y, z);
+  }
   double acc[TLBM_NSUMS];
   step_cell<COLL, OBS, SIDE, REPORT, STORAGE, HEAT, NT, ROWS>(
       a, static_cast<const T*>(a.fin), static_cast<T*>(a.fout), static_cast<int>(x),
