@@ -9,6 +9,12 @@ residuals, FP64 sums) every 100 steps.  A "step" is one lattice update of the wh
 value = whole-grid MLUPS over the timed steps (report steps included) with FP32 populations;
 the FP16 shifted-storage variant is timed the same way and reported under "variants" (BF16 with
 --storage f32,f16,bf16: it misses the 1% Strouhal/drag bar against the FP64 oracle at Re 300).
+The 16-bit variants warm up for at least VARIANT_WARMUP steps (their "warmup" field): when the
+16-bit step starts after other work, the GPU's engine clock drops under its power management
+(measured with GRBM_GUI_ACTIVE per dispatch: from ~2.3 GHz to 1.65-1.9 GHz for the first ~40
+launches, 0.45-0.52 ms each against 0.41-0.42 ms once settled), and the 16-bit step, whose VALU
+work is nearly as long as its memory time, slows with it; a few warm-up steps would time that
+transient instead of the step.
 Each variant also reports its pattern ceiling: the step's 19-load/19-store access pattern with
 no arithmetic (tlbm_pull_copy) on the same buffers.
 roofline: the quiet FP32 step kernel, 152 algorithmic bytes per cell (19 x 4 B read + 19 x 4 B
@@ -79,8 +85,12 @@ class Events:
         return ms.value
 
 
+VARIANT_WARMUP = 100
+
+
 def time_gpu(storage, steps, warmup, block_x=0):
-    """Times `steps` steps (report every 100th) and the quiet-kernel mean from HIP events."""
+    """Times `steps` steps (report every 100th) and the quiet-kernel mean from HIP events, after
+    `warmup` untimed steps."""
     import tlbm
     from tlbm._capi import check, lib
     L = lib()
@@ -131,6 +141,7 @@ def time_gpu(storage, steps, warmup, block_x=0):
     ceiling_ms = time_ceiling(s, stream)
     out = {
         "storage": storage,
+        "warmup": warmup,
         "mlups": n * steps / (total_ms * 1e-3) / 1e6,
         "ms_per_step": total_ms / steps,
         "wall_ms_per_step": wall * 1e3 / steps,
@@ -528,8 +539,10 @@ def main():
         ap.error("--sweep-steps must be a positive multiple of 100")
     storages = [s for s in args.storage.split(",") if s]
     results = {}
+    head_storage = "f32" if "f32" in storages else storages[0]
     for st in storages:
-        results[st] = time_gpu(st, args.steps, args.warmup, args.block)
+        warm = args.warmup if st == head_storage else max(args.warmup, VARIANT_WARMUP)
+        results[st] = time_gpu(st, args.steps, warm, args.block)
         r = results[st]
         print(f"[bench] {st}: {r['mlups']:.0f} MLUPS, quiet kernel {r['quiet_kernel_ms']:.3f} ms "
               f"= {r['achieved_gbs']:.0f} GB/s; the same access pattern without arithmetic "
@@ -600,6 +613,7 @@ def main():
                      "algorithmic_bytes": BYTES_PER_CELL[head["storage"]] * NX * NY * NZ},
         "cpu_baseline": cpu,
         "variants": {st: {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+                          "warmup": r["warmup"],
                           "quiet_kernel_ms": round(r["quiet_kernel_ms"], 4),
                           "achieved_gbs": round(r["achieved_gbs"], 1),
                           "roofline_frac": round(r["achieved_gbs"] / PEAK_GBS, 4),
