@@ -183,9 +183,10 @@ int tlbm_copy(void* dst, const void* src, size_t bytes, void* stream);
 /* The step's data movement without its arithmetic: f_out[q*qstride + idx] =
  * f_in[q*qstride + src_q(idx)] for the 19 populations, src_q the pull source of lbm.py:410-415
  * with x, y and z periodic (collisionless streaming), launched and addressed exactly as the
- * step's loads and stores (params: nx, ny, nz, qstride; storage picks 4- or 2-byte elements,
- * copied raw).  No counterpart in the reference: the bandwidth it reaches is the ceiling of the
- * step's 19-read/19-write access pattern on the device. */
+ * quiet step's loads and stores, with the same cache policies (params: nx, ny, nz, qstride,
+ * tile_rows, xcd_group; storage picks 4- or 2-byte elements, copied raw).  No counterpart in
+ * the reference: the bandwidth it reaches is the ceiling of the step's 19-read/19-write access
+ * pattern on the device. */
 int tlbm_pull_copy(const void* f_in, void* f_out, const tlbm_params* params, int storage,
                    void* stream);
 
