@@ -155,8 +155,41 @@ def test_tiled_pull_copy_moves_what_the_planes_copy_moves(storage, dims, k):
         outs.append(raw.reshape(19, lat.n) if tiles == 0
                     else tlbm.untile_populations(raw, lat, tiles))
     assert np.array_equal(outs[0], outs[1])
-    # and the copy is the pull: plane q of cell idx comes from cell idx - c_q (periodic)
-    q, x, y, z = 7, 0, 3, 2
+    # and the copy is the pull: plane q of cell (x, y, z) comes from (x, y, z) - c_q (periodic)
+    _check_pull(outs[0], planes, dims)
+
+
+def _check_pull(out, planes, dims):
+    import tlbm
     nx, ny, nz = dims
-    src = (((z - tlbm.CZ[q]) % nz) * ny + (y - tlbm.CY[q]) % ny) * nx + (x - tlbm.CX[q]) % nx
-    assert outs[0][q, (z * ny + y) * nx + x] == planes[q, src]
+    for q in range(19):
+        want = np.roll(planes[q].reshape(nz, ny, nx), (tlbm.CZ[q], tlbm.CY[q], tlbm.CX[q]),
+                       axis=(0, 1, 2))
+        assert np.array_equal(out[q].reshape(nz, ny, nx), want), q
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_large_pull_copy_is_the_pull(storage):
+    """512 x 256 x 32 in tiles of 4 rows: population buffers beyond 512 MB (FP32), so the copy
+    takes the quiet step's large-lattice build (non-temporal stores and unshifted loads, rows
+    addressed per wave); every plane equals the periodic pull of the input."""
+    import tlbm
+    from tlbm import _capi
+    dims, k = (512, 256, 32), 4
+    print(f"[tiles] large pull copy {storage} {dims} {k}", flush=True)
+    L = _capi.lib()
+    lat = tlbm.D3Q19(*dims)
+    rng = np.random.default_rng(12)
+    bits = np.uint32 if storage == "f32" else np.uint16
+    planes = rng.integers(0, 1 << (8 * np.dtype(bits).itemsize), (19, lat.n),
+                          dtype=np.uint64).astype(bits)
+    src = tlbm.tile_populations(planes, lat, k)
+    a = tlbm.empty((src.size,), storage)
+    a.copy_from(src.view(a.dtype))
+    b = tlbm.empty((src.size,), storage)
+    p = tlbm.make_params(lat, None, tlbm.Cylinder(-100.0, -100.0, 1.0), tlbm.Flow(0.05, 1e9, 0.0),
+                         lat.n, k)
+    _capi.check(L.tlbm_pull_copy(a.ptr, b.ptr, ctypes.byref(p), tlbm.STORAGES[storage], None),
+                "pull copy")
+    tlbm.sync()
+    _check_pull(tlbm.untile_populations(b.numpy().view(bits), lat, k), planes, dims)
