@@ -256,7 +256,8 @@ def time_sweep(storage, steps=5000, graph_steps=100, seq_steps=20, launch=None):
     """The parameter sweep: 64 seeded TRT + Smagorinsky channels (nx in [64, 192], ny = nz =
     nx / 2, Re in [100, 1000], cylinder radius in [4, 12], u0 = 0.08, seeded 1e-3 noise) stepped
     `steps` times by one persistent launch per step, replayed from a hipGraph of `graph_steps`
-    steps with a report step at its end; launch shape from the autotuner.  For comparison, the
+    steps with a report step at its end; the BatchSolver's default population layout (tiles of
+    rows at FP32, padded planes at FP16), launch shape from the autotuner.  For comparison, the
     same domains stepped one launch per domain (one Solver each) for `seq_steps` steps.
     `launch` = (block, persistent grid, chunk group) pins the launch shape instead."""
     import tlbm
@@ -288,6 +289,7 @@ def time_sweep(storage, steps=5000, graph_steps=100, seq_steps=20, launch=None):
     ms = ev.elapsed(e0, e1)
     mons = b.monitors(graph_steps)
     cells = b.cells
+    tiles = sorted(set(b.tile_rows))
     b.close()
     del b
     # one launch per domain per step
@@ -311,6 +313,7 @@ def time_sweep(storage, steps=5000, graph_steps=100, seq_steps=20, launch=None):
             "achieved_gbs": bpc * cells * steps / (ms * 1e-3) / 1e9,
             "launch": {"block": block, "persistent_grid": grid, "xcd_chunk_group": group,
                        "tuned_ms": round(tune_ms, 4)},
+            "tile_rows": tiles,
             "one_launch_per_domain_mlups": cells * seq_steps / (seq_ms * 1e-3) / 1e6,
             "drag_range": [min(m["drag"] for m in mons), max(m["drag"] for m in mons)]}
 

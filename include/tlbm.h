@@ -258,8 +258,9 @@ int tlbm_adjoint_step(const tlbm_params* params, const float* f_t, const float* 
  * independent lattices that share one specialisation (collision, obstacle, side, storage of
  * `handle`; its diag is ignored, both variants are available) and differ in size and runtime
  * arguments (params[d]).  Their populations live in one buffer: domain d at element offset
- * elem_offset(d), q-major with plane stride params[d].qstride (a value below the domain's
- * nx*ny*nz means nx*ny*nz: packed), the domains one after another; packed
+ * elem_offset(d), laid out as a single lattice of params[d] (tlbm_population_elems): q-major
+ * with plane stride params[d].qstride (a value below the domain's nx*ny*nz means nx*ny*nz:
+ * packed), or tiles of params[d].tile_rows rows, the domains one after another; packed
  * per-cell arrays (report fields) hold domain d at cell_offset(d).  One launch steps every
  * domain: a persistent grid walks chunks of block_x consecutive cells of one domain. */
 typedef struct tlbm_batch tlbm_batch;

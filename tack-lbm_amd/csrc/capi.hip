@@ -502,13 +502,13 @@ int batch_build(tlbm_batch* b) {
     const tlbm_params& p = b->params[d];
     const uint64_t n = static_cast<uint64_t>(p.nx) * p.ny * p.nz;
     tlbm::BatchDomain& D = b->dom[d];
-    // a.ns = the domain's plane stride (>= n)
+    // the domain's layout as a single lattice's: q-major planes of qstride, or tiles of rows
     D.a = make_args(p, b->spec.obstacle, b->spec.storage == TLBM_F32 ? 4u : 2u);
     D.elem_off = elem;
     D.cell_off = static_cast<uint32_t>(cell);
     D.chunk_off = static_cast<uint32_t>(chunk);
     b->chunk_off[d] = static_cast<uint32_t>(chunk);
-    elem += 19 * static_cast<uint64_t>(p.qstride);
+    elem += population_elems(p);
     cell += n;
     chunk += (n + b->block - 1) / b->block;
   }
@@ -967,9 +967,6 @@ int tlbm_batch_create(int handle, const tlbm_params* params, int ndomains, tlbm_
     const int64_t n = static_cast<int64_t>(p.nx) * p.ny * p.nz;
     if (p.qstride < n) p.qstride = n;  // a plane stride below the cell count means "packed"
     if (int rc = check_params(&p, d.obstacle)) return rc;
-    if (p.tile_rows != 0)
-      return fail(TLBM_ERR_INVALID, "batched domains use q-major planes (domain %d has tile_rows %d)",
-                  i, p.tile_rows);
   }
   tlbm_batch* b = new tlbm_batch();
   b->handle = handle - d.diag;

@@ -73,7 +73,8 @@ def tune_batch(batch, reps=5, use_cache=True, candidates=None):
     cus = ctypes.c_int()
     check(L.tlbm_device_cus(ctypes.byref(cus)), "device CUs")
     key = (f"batch|{device_key()}|h{batch.handle}|{len(batch.domains)}dom|{batch.cells}cells|"
-           + ",".join(f"{d.nx}x{d.ny}x{d.nz}" for d in batch.domains[:4]))
+           f"{batch.elements}elems|" + ",".join(f"{d.nx}x{d.ny}x{d.nz}" for d in batch.domains[:4])
+           + "|t" + ",".join(str(k) for k in getattr(batch, "tile_rows", [])[:4]))
     table = _load() if use_cache else {}
     if key in table:
         block, grid, ms = table[key][:3]

@@ -18,7 +18,7 @@ from . import _capi
 from ._capi import check, lib
 from .lbm import (COLLISIONS, FIELDS, LidY, PeriodicY, SlipY, Box, Cylinder, Sphere,
                   STORAGES, D3Q19, Flow, HostBuffer, decode_populations, make_params,
-                  kernel_handle, sync, zeros)
+                  kernel_handle, sync, untile_populations, zeros)
 
 _OBSTACLES = {"cylinder": (_capi.CYLINDER, Cylinder), "sphere": (_capi.SPHERE, Sphere),
               "box": (_capi.BOX, Box)}
@@ -43,16 +43,19 @@ class Domain:
 class BatchSolver:
     """All domains of a sweep, one specialisation (collision x obstacle x side x storage).
 
-    Domain d's populations sit at ``offsets(d)[0]`` in ``f``, q-major with plane stride
-    ``plane_strides[d]``: its cell count rounded up to 64 plus ``plane_pad`` (0: packed).
-    Measured on MI355X over the 64-channel sweep, a 256-cell pad streams 17% faster (FP32) and
-    12% faster (FP16) than packed planes.
+    Domain d's populations sit at ``offsets(d)[0]`` in ``f``.  ``tile_rows`` = 0: q-major with
+    plane stride ``plane_strides[d]``, its cell count rounded up to 64 plus ``plane_pad`` (0:
+    packed); measured on MI355X over the 64-channel sweep, a 256-cell pad streams 17% faster
+    (FP32) and 12% faster (FP16) than packed planes.  ``tile_rows`` = k > 0: tiles of k rows in
+    every domain (19 planes of k nx cells per tile, as a Solver's ``tile_rows``); "auto" (the
+    default): per domain ``batch_tile_rows`` (tiles at FP32, planes at FP16/BF16); a sequence:
+    one value per domain.  ``tile_rows[d]`` holds each domain's choice.
     """
 
     def __init__(self, domains, collision="trt", obstacle="cylinder", lateral="periodic",
                  u0=0.08, storage="f32", cs=None, magic=None, seed_amp=0.05, noise_amp=0.0,
                  noise_seed=0, lid_u=(0.04, 0.0, 0.0), sponge_strength=1.0, stream=None,
-                 fields=False, plane_pad=256):
+                 fields=False, plane_pad=256, tile_rows="auto"):
         if storage not in STORAGES:
             raise ValueError(f"storage must be one of {sorted(STORAGES)}")
         self.domains = list(domains)
@@ -68,7 +71,7 @@ class BatchSolver:
                                     STORAGES[storage])
         self._omega = zeros((1,))
         params = (_capi.Params * len(self.domains))()
-        self.phys, self.geo, self.plane_strides = [], [], []
+        self.phys, self.geo, self.plane_strides, self.tile_rows = [], [], [], []
         for i, d in enumerate(self.domains):
             nu = u0 * d.diameter / d.reynolds
             kw = {}
@@ -84,10 +87,22 @@ class BatchSolver:
                    "box": lambda: Box(x0, y0, z0, r, r, 0.3 * d.nz)}[obstacle]()
             flow = Flow(u0, 0.75 * d.nx if d.sponge_start is None else d.sponge_start,
                         sponge_strength, self._omega, side)
+            if isinstance(tile_rows, str):
+                if tile_rows != "auto":
+                    raise ValueError(f"tile_rows {tile_rows!r} is not 0, a power of two or 'auto'")
+                k = batch_tile_rows(d.nx, storage)
+            elif np.ndim(tile_rows):
+                k = int(tile_rows[i])
+            else:
+                k = int(tile_rows)
+            if k < 0 or k & (k - 1):
+                raise ValueError(f"tile_rows {k} is not 0, a power of two or 'auto'")
             # plane stride: packed, or padded past a 64-cell boundary by `plane_pad` cells
-            qs = d.n if plane_pad <= 0 else (d.n + 63) // 64 * 64 + int(plane_pad)
-            params[i] = make_params(D3Q19(d.nx, d.ny, d.nz), phys, geo, flow, qs)
+            # (q-major planes only; tiles have none)
+            qs = d.n if plane_pad <= 0 or k else (d.n + 63) // 64 * 64 + int(plane_pad)
+            params[i] = make_params(D3Q19(d.nx, d.ny, d.nz), phys, geo, flow, qs, k)
             self.plane_strides.append(qs)
+            self.tile_rows.append(k)
             self.phys.append(phys)
             self.geo.append(geo)
         h = ctypes.c_void_p()
@@ -188,8 +203,14 @@ class BatchSolver:
         """Domain d's populations as float32, shaped (19, n)."""
         sync(self.stream)
         e, _ = self.offsets(d)
-        n, qs = self.domains[d].n, self.plane_strides[d]
-        raw = self.f[0].numpy()[e:e + 19 * qs].reshape(19, qs)[:, :n]
+        e1, _ = self.offsets(d + 1)
+        dom, k = self.domains[d], self.tile_rows[d]
+        flat = self.f[0].numpy()[e:e1]
+        if k:
+            raw = untile_populations(flat, D3Q19(dom.nx, dom.ny, dom.nz), k)
+        else:
+            qs = self.plane_strides[d]
+            raw = flat[:19 * qs].reshape(19, qs)[:, :dom.n]
         return decode_populations(raw, self.storage)
 
     def close(self):
@@ -205,6 +226,23 @@ class BatchSolver:
             self.close()
         except Exception:
             pass
+
+
+def batch_tile_rows(nx, storage="f32"):
+    """Rows per tile of a batched domain with rows of nx cells (0: q-major planes).  FP32: the
+    largest power of two k with k nx <= 4096 cells (16 KiB of a plane per tile).  Measured on
+    MI355X over the 64-channel TRT sweep (nx in [64, 192]): 1.158-1.160 ms per step with 16 to
+    32 KiB per plane (or 16 or 32 rows for every domain), 1.237 ms with 8 KiB, 1.205-1.212 ms
+    with padded q-major planes.  FP16/BF16: 0, since tiles make the 16-bit loads of rows that
+    are not whole waves form a 64-bit address per lane (0.744-0.768 ms against 0.705-0.721)."""
+    if storage != "f32":
+        return 0
+    if nx > 4096:
+        return 0
+    k = 1
+    while 2 * k * nx <= 4096:
+        k *= 2
+    return k
 
 
 def sweep_domains(count=64, seed=0, nx_range=(64, 192), re_range=(100.0, 1000.0),

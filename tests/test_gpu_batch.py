@@ -26,10 +26,29 @@ def _domains():
 
 
 def _view(raw, batch, d):
-    """Domain d's (19, n) populations inside a raw batch buffer (plane stride of the batch)."""
+    """Domain d's (19, n) populations inside a raw batch buffer (q-major planes of the batch's
+    plane stride, or tiles of rows: then a copy)."""
+    import tlbm
     e, _ = batch.offsets(d)
-    n, qs = batch.domains[d].n, batch.plane_strides[d]
-    return raw[e:e + 19 * qs].reshape(19, qs)[:, :n]
+    dom, k = batch.domains[d], batch.tile_rows[d]
+    if k:
+        lat = tlbm.D3Q19(dom.nx, dom.ny, dom.nz)
+        return tlbm.untile_populations(raw[e:e + tlbm.population_elems(lat, None, k)], lat, k)
+    qs = batch.plane_strides[d]
+    return raw[e:e + 19 * qs].reshape(19, qs)[:, :dom.n]
+
+
+def _put(raw, batch, d, planes):
+    """Write domain d's (19, n) populations into a raw batch buffer in the batch's layout."""
+    import tlbm
+    dom, k = batch.domains[d], batch.tile_rows[d]
+    if k:
+        e, _ = batch.offsets(d)
+        lat = tlbm.D3Q19(dom.nx, dom.ny, dom.nz)
+        tiled = tlbm.tile_populations(planes, lat, k)
+        raw[e:e + tiled.size] = tiled
+    else:
+        _view(raw, batch, d)[:] = planes
 
 
 def _random_state(batch, seed):
@@ -41,7 +60,7 @@ def _random_state(batch, seed):
         u = 0.05 * rng.standard_normal((3, dom.n))
         f = np.stack([O.feq(q, rho, *u) for q in range(19)]) + 1e-3 * rng.standard_normal(
             (19, dom.n))
-        _view(raw, batch, d)[:] = tlbm.encode_populations(f.astype(np.float32), batch.storage)
+        _put(raw, batch, d, tlbm.encode_populations(f.astype(np.float32), batch.storage))
     batch.f[0].copy_from(raw)
     return raw
 
@@ -62,14 +81,23 @@ def _single_step(batch, d, raw, report):
     return out.numpy(), sums.numpy.copy()
 
 
+# population layouts: padded planes, packed planes, tiles of 4 rows in every domain, and the
+# default (tiles of batch_tile_rows rows at FP32, padded planes at FP16/BF16)
+LAYOUTS = {"pad256": (256, 0), "packed": (0, 0), "tiles4": (256, 4), "auto": (256, "auto")}
+
+
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
 @pytest.mark.parametrize("collision", ["bgk", "trt", "les"])
-@pytest.mark.parametrize("plane_pad", [256, 0])
-def test_batch_step_matches_single_lattice_and_oracle(storage, collision, plane_pad):
+@pytest.mark.parametrize("layout", sorted(LAYOUTS))
+def test_batch_step_matches_single_lattice_and_oracle(storage, collision, layout):
     import tlbm
     from tlbm.batch import BatchSolver
+    plane_pad, tile_rows = LAYOUTS[layout]
     b = BatchSolver(_domains(), collision=collision, storage=storage, cs=0.1, magic=0.25,
-                    plane_pad=plane_pad)
+                    plane_pad=plane_pad, tile_rows=tile_rows)
+    if layout == "auto":
+        assert b.tile_rows == [tlbm.batch.batch_tile_rows(d.nx, storage) for d in b.domains]
+        assert any(b.tile_rows) == (storage == "f32")
     raw = _random_state(b, seed=5)
     b.step(report=True)
     tlbm.sync()
@@ -193,7 +221,8 @@ def test_large_batch_domain_steps_as_the_single_lattice():
                     lateral="lid", storage="f32", noise_amp=1e-3)
     b.step()
     tlbm.sync()
-    got = _view(b.f[0].numpy(), b, 1)
+    got = b.populations(1)
+    assert b.tile_rows[1] == 16  # 16 rows of 256 cells: 16 KiB of a plane per tile
     s = tlbm.Solver(256, 256, 300, collision="les", obstacle="sphere", reynolds=3000.0,
                     diameter=32.0, center=(96.0, 128.0, 150.0), lateral="lid", noise_amp=1e-3,
                     qstride=b.plane_strides[1])

@@ -85,18 +85,12 @@ def test_default_tile_rows():
     assert default_tile_rows(128, 64, 64) == 0           # below 2^20 cells
 
 
-def test_q_major_only_entry_points_reject_tiles():
-    """The batch and the adjoint take q-major planes: a tiled params is refused before any
-    allocation or launch."""
+def test_adjoint_rejects_tiles():
+    """The adjoint takes q-major planes: a tiled params is refused before any launch."""
     from tlbm import _capi
     lib = _capi.lib()
     p = _params((8, 6, 4), 192, 2)
     p.tau0 = 0.56
-    h = ctypes.c_int(-1)
-    lib.tlbm_kernel_lookup(ctypes.byref(_capi.Spec(1, 0, 0, 0, 0, 0)), ctypes.byref(h))
-    out = ctypes.c_void_p()
-    assert lib.tlbm_batch_create(h, ctypes.byref(p), 1, ctypes.byref(out)) != 0
-    assert "q-major" in _capi.last_error() and not out.value
     a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
     org = (ctypes.c_int32 * 3)(0, 0, 0)
     size = (ctypes.c_int32 * 3)(1, 1, 1)
@@ -104,3 +98,15 @@ def test_q_major_only_entry_points_reject_tiles():
     assert lib.tlbm_adjoint_step(ctypes.byref(p), a, b, a, b, 1.0, 1.0, org, size,
                                  ctypes.c_void_p(0x4000), 0, None) != 0
     assert "q-major" in _capi.last_error()
+
+
+def test_batch_tile_rows():
+    """Batched domains: tiles of 16 KiB per plane at FP32 (the largest power of two k with
+    k nx <= 4096), q-major planes for 16-bit storage and for rows longer than a tile."""
+    from tlbm.batch import batch_tile_rows
+    assert [batch_tile_rows(nx) for nx in (64, 100, 128, 130, 192, 256, 4096, 5000)] == \
+        [64, 32, 32, 16, 16, 16, 1, 0]
+    for nx in (64, 100, 4096):
+        k = batch_tile_rows(nx)
+        assert k * nx <= 4096 < 2 * k * nx
+    assert batch_tile_rows(128, "f16") == batch_tile_rows(128, "bf16") == 0
