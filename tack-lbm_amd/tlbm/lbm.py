@@ -556,13 +556,18 @@ def default_qstride(n):
 
 def default_tile_rows(nx, ny, nz, storage="f32"):
     """Rows per population tile for a Solver of nx x ny x nz cells (0: q-major planes): the
-    largest power of two k with k nx cells of one plane in at most 8 KiB (k nx <= 2048 cells at
-    FP32, 4096 at FP16/BF16).  Measured on MI355X at 512x256x256 (LES, quiet step): FP32 0.842 ms
-    with tiles of 4 or 16 rows, 0.851 with 8, 0.859 with 2, against 0.891-0.904 ms for q-major
-    planes; FP16 0.443-0.445 ms with 8 rows, 0.446-0.448 with 4, 0.458-0.461 for planes.  Only
-    for large lattices whose rows are whole waves (nx a multiple of 64), where the 16-bit step
-    addresses a wave's source row once for all its lanes."""
-    per = 8192 // (4 if storage == "f32" else 2)
+    largest power of two k with k nx cells of one plane in at most 32 KiB at FP32 (k nx <= 8192
+    cells) and 8 KiB at FP16/BF16 (k nx <= 4096).  Measured on MI355X (LES, sphere, lid, quiet
+    step, FP32 with non-temporal device-scope stores): 512x256x256 0.806 ms with 16 rows against
+    0.814-0.819 with 4, 0.827 with 8 and 0.834 with 32; 256x256x256 0.403 ms with 32 rows against
+    0.414-0.419 with 4 to 16; 1024x256x128 0.802-0.805 with 8 against 0.820-0.826 with 2 or 4;
+    512x128x128 0.2045 with 16 against 0.206-0.209; FP16 512x256x256 0.417-0.418 ms with 8, 16
+    or 32 rows (0.423 with 4, 0.451 with 64), 1024x256x128 0.415-0.419 with 4 against 0.421 with
+    8 and 0.430 with 16.  Q-major planes: 0.891-0.904 ms (FP32), 0.458-0.461 (FP16) at
+    512x256x256 before the per-wave row addressing.  Only for large lattices whose rows are
+    whole waves (nx a multiple of 64), where the step addresses a wave's source rows once for
+    all its lanes."""
+    per = 8192 if storage == "f32" else 4096
     if nx * ny * nz < (1 << 20) or nx % 64 or nx > per:
         return 0
     k = 1

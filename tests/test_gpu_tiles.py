@@ -78,7 +78,7 @@ def test_default_tiles_on_a_large_lattice(storage, dims):
     import tlbm
     print(f"[tiles] default tiles {storage} {dims}", flush=True)
     k = tlbm.default_tile_rows(*dims, storage)
-    assert k == (8 if storage == "f32" else 16)
+    assert k == (32 if storage == "f32" else 16)
     a, b = _pair("les", "sphere", "lid", dims, k, storage)
     for s in (a, b):
         for i in range(4):

@@ -69,18 +69,22 @@ def test_population_elems_matches_the_library():
 
 
 def test_default_tile_rows():
-    """Tiles whose plane segment holds at most 8 KiB, only for lattices of 2^20 cells or more
-    with rows of whole waves."""
+    """Tiles whose plane segment holds at most 32 KiB at FP32 and 8 KiB at FP16/BF16, only for
+    lattices of 2^20 cells or more with rows of whole waves."""
     from tlbm import default_tile_rows
-    assert default_tile_rows(512, 256, 256) == 4
+    assert default_tile_rows(512, 256, 256) == 16
     assert default_tile_rows(512, 256, 256, "f16") == 8
     assert default_tile_rows(512, 256, 256, "bf16") == 8
-    assert default_tile_rows(384, 256, 256) == 4
-    assert default_tile_rows(256, 64, 64) == 8
-    assert default_tile_rows(1024, 128, 128) == 2
-    assert default_tile_rows(2048, 32, 32) == 1
-    assert default_tile_rows(4096, 32, 32) == 0          # a row beyond 8 KiB at FP32
+    assert default_tile_rows(384, 256, 256) == 16
+    assert default_tile_rows(256, 256, 256) == 32
+    assert default_tile_rows(256, 64, 64) == 32
+    assert default_tile_rows(1024, 256, 128) == 8
+    assert default_tile_rows(1024, 256, 128, "f16") == 4
+    assert default_tile_rows(2048, 32, 32) == 4
+    assert default_tile_rows(8192, 16, 16) == 1
+    assert default_tile_rows(16384, 8, 8) == 0           # a row beyond 32 KiB at FP32
     assert default_tile_rows(4096, 32, 32, "f16") == 1
+    assert default_tile_rows(8192, 16, 16, "f16") == 0   # a row beyond 8 KiB at FP16
     assert default_tile_rows(480, 256, 256) == 0         # rows not whole waves
     assert default_tile_rows(128, 64, 64) == 0           # below 2^20 cells
 
