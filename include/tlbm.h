@@ -18,7 +18,8 @@
  *     counterpart in the reference): with row r = z*ny + y and T = k*nx, population q of cell
  *     (x, r) lives at f[(r/k)*19*T + q*T + (r%k)*nx + x], i.e. each tile of k rows holds its 19
  *     planes together; tlbm_population_elems gives the buffer length.  The temperature planes
- *     g stay q-major with qstride in both layouts.
+ *     g follow the same layout with 7 planes: g[q*qstride + idx], or tiles of k rows holding
+ *     their 7 planes of k*nx cells together.
  */
 #ifndef TLBM_H
 #define TLBM_H
@@ -73,8 +74,8 @@ typedef struct {
   int32_t nx, ny, nz;      /* D3Q19(nx, ny, nz), lbm.py:42-44                           */
   int32_t block_x;         /* threads per block of the step: 64, 128, 192 or 256; 0: the
                               handle's default (tlbm_set_launch, else 256)                */
-  int64_t qstride;         /* elements between population planes, >= nx*ny*nz (with
-                              tile_rows > 0: the temperature planes only)                 */
+  int64_t qstride;         /* elements between population (and temperature) planes,
+                              >= nx*ny*nz; not used for the layout when tile_rows > 0     */
   int32_t tile_rows;       /* 0: q-major population planes; k > 0 (a power of two): tiles
                               of k rows, 19 planes of k*nx cells each (see the layout note) */
   int32_t xcd_group;       /* quiet steps: blocks of consecutive cells go to the XCDs in
@@ -212,8 +213,10 @@ typedef struct {
 } tlbm_heat;
 
 /* lbm_step with heat enabled (lbm.py:397-513): the flow step of tlbm_step (same arguments and
- * report behaviour) plus the temperature step from g_in to g_out (7 planes of qstride floats,
- * distinct buffers).  Needs a handle looked up with spec.heat = 1. */
+ * report behaviour) plus the temperature step from g_in to g_out (distinct buffers of 7 planes
+ * of qstride floats, or with tile_rows = k > 0 tiled as the populations: 7 planes of k*nx cells
+ * per tile of k rows, 7*k*nx*ceil(ny*nz/k) floats).  Needs a handle looked up with
+ * spec.heat = 1. */
 int tlbm_step_heat(int handle, const void* f_in, void* f_out, const float* g_in, float* g_out,
                    const tlbm_params* params, const tlbm_heat* heat, const tlbm_fields* fields,
                    double* sums, void* stream);

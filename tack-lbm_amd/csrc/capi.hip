@@ -273,6 +273,11 @@ uint64_t population_elems(const tlbm_params& p) {
   return 19ull * k * static_cast<uint64_t>(p.nx) * ((rows + k - 1) / k);
 }
 
+// Elements of one temperature buffer: 7 planes of qstride, or 7 planes of k nx per tile of k rows.
+uint64_t heat_population_elems(const tlbm_params& p) {
+  return population_elems(p) / 19 * 7;
+}
+
 int check_params(const tlbm_params* p, int obstacle = -1) {
   if (!p) return fail(TLBM_ERR_INVALID, "params is NULL");
   if (obstacle == TLBM_FIELD && !p->phi)
@@ -334,7 +339,10 @@ tlbm::StepArgs make_args(const tlbm_params& p, int obstacle, uint32_t elem_bytes
   const bool tiled = p.tile_rows > 0;
   a.ns = tiled ? static_cast<uint64_t>(p.tile_rows) * static_cast<uint64_t>(p.nx)
                : static_cast<uint64_t>(p.qstride);
-  a.gns = static_cast<uint64_t>(p.qstride);
+  // the temperature planes: tiled as the populations (7 planes per tile), or q-major with qstride
+  a.gns = tiled ? a.ns : static_cast<uint64_t>(p.qstride);
+  a.g_tile_stride = tiled ? 7ull * a.ns : 0ull;
+  a.g_off32 = heat_population_elems(p) * 4 <= 0xFFFFFFFFull ? 1u : 0u;
   a.tile_stride = tiled ? 19ull * a.ns : 0ull;
   a.tile_log2 = 31u;
   a.tile_mask = 0xFFFFFFFFu;
@@ -767,7 +775,7 @@ int step_impl(int handle, const Decoded& d, const void* f_in, void* f_out, const
   a.fout = f_out;
   // the temperature step adds 7 FP32 planes (g) to the 19 population planes
   a.stream_stores = stream_stores(population_elems(*params) +
-                                      (heat ? 7 * static_cast<uint64_t>(params->qstride) : 0),
+                                      (heat ? heat_population_elems(*params) : 0),
                                   heat ? TLBM_F32 : d.storage) ? 1u : 0u;
   if (heat) {
     if (!g_in || !g_out) return fail(TLBM_ERR_INVALID, "g_in or g_out is NULL");

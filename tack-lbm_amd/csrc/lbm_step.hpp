@@ -176,7 +176,10 @@ struct StepArgs {
   const void* fin;
   void* fout;
   uint64_t ns;               // population plane stride (elements): qstride, or k nx in tiles
-  uint64_t gns;              // temperature plane stride (elements): qstride
+  uint64_t gns;              // temperature plane stride (elements): qstride, or k nx in tiles
+  uint64_t g_tile_stride;    // temperature tiles: 7 k nx (7 planes of k nx per tile of k rows,
+                             // the populations' tiles), 0 for q-major planes
+  uint32_t g_off32;          // 1: the temperature buffer lies within 2^32 bytes
   // Population layout.  Row r = z ny + y starts at element row_elem(r) of plane 0:
   //   q-major planes: tile_stride = 0, tile_log2 = 31, tile_mask = all ones -> r nx;
   //   tiles of k rows: tile_stride = 19 k nx, tile_log2 = log2 k, tile_mask = k - 1 ->
@@ -757,19 +760,24 @@ __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z
   using Obs = ObstacleT<OBS>;
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const uint32_t snx = static_cast<uint32_t>(nx);
-  const uint32_t plane = snx * static_cast<uint32_t>(ny);
   const int xs[3] = {x == nx - 1 ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};
   const int ys[3] = {y == ny - 1 ? 0 : y + 1, y, y == 0 ? ny - 1 : y - 1};
   const int zs[3] = {z == nz - 1 ? 0 : z + 1, z, z == 0 ? nz - 1 : z - 1};
   const float* __restrict__ gin = a.g_in;
+  // first element of row r (= z ny + y) in temperature plane 0: q-major planes (r nx) or the
+  // populations' tiles of rows with 7 planes per tile
+  auto g_row = [&](uint32_t r) {
+    return static_cast<uint64_t>(r >> a.tile_log2) * a.g_tile_stride + (r & a.tile_mask) * snx;
+  };
+  const uint64_t g_own = g_row(static_cast<uint32_t>(z) * ny + y) + static_cast<uint32_t>(x);
   float g[kQ7];
-  uint32_t nidx[kQ7];
+  uint32_t nidx[kQ7];  // source cells (t_solid, q_solid)
   static_for<0, kQ7>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     constexpr int cx = kCX7[q], cy = kCY7[q], cz = kCZ7[q];
-    nidx[q] = static_cast<uint32_t>(zs[1 + cz]) * plane + static_cast<uint32_t>(ys[1 + cy]) * snx +
-              static_cast<uint32_t>(xs[1 + cx]);
-    g[q] = gin[static_cast<uint64_t>(q) * a.gns + nidx[q]];
+    const uint32_t r = static_cast<uint32_t>(zs[1 + cz]) * ny + static_cast<uint32_t>(ys[1 + cy]);
+    nidx[q] = r * snx + static_cast<uint32_t>(xs[1 + cx]);
+    g[q] = gin[static_cast<uint64_t>(q) * a.gns + g_row(r) + static_cast<uint32_t>(xs[1 + cx])];
   });
   const bool wrap = ((a.wrap & 1) && (x == 0 || x == nx - 1)) ||
                     ((a.wrap & 2) && (y == 0 || y == ny - 1)) ||
@@ -783,7 +791,7 @@ __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z
     static_for<0, kQ7>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       constexpr int cx = kCX7[q], cy = kCY7[q], cz = kCZ7[q];
-      own[q] = gin[static_cast<uint64_t>(opp7(q)) * a.gns + idx];
+      own[q] = gin[static_cast<uint64_t>(opp7(q)) * a.gns + g_own];
       wall[q] = q == 0 ? here : Obs::solid(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]);
       hot[q] = wall[q] != 0.0f ? Obs::sink(a, xs[1 + cx], ys[1 + cy], zs[1 + cz]) : 0.0f;
     });
@@ -816,17 +824,18 @@ __device__ __forceinline__ void heat_cell(const StepArgs& a, int x, int y, int z
       const float eqs = kW7[q] * temp_s * (1.0f + 4.0f * (cx * ex.uxs + cy * ex.uys + cz * ex.uzs));
       gout[q] = eqs + keep * (post - eq);
     });
-    if (NT && a.gns * 4 * kQ7 <= 0xFFFFFFFFull) {
-      // non-temporal stores at device scope (buffer_store), the 7 planes within 2^32 bytes
+    if (NT && a.g_off32) {
+      // non-temporal stores at device scope (buffer_store), the buffer within 2^32 bytes
       const auto rg = plane_rsrc(a.g_out);
       static_for<0, kQ7>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        buffer_store<float, true>(gout[q], rg, idx * 4u, static_cast<uint32_t>(q * a.gns * 4u));
+        buffer_store<float, true>(gout[q], rg, static_cast<uint32_t>(g_own) * 4u,
+                                  static_cast<uint32_t>(q * a.gns * 4u));
       });
     } else {
       static_for<0, kQ7>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        float* dst = &a.g_out[static_cast<uint64_t>(q) * a.gns + idx];
+        float* dst = &a.g_out[static_cast<uint64_t>(q) * a.gns + g_own];
         if constexpr (NT) __builtin_nontemporal_store(gout[q], dst);
         else *dst = gout[q];
       });

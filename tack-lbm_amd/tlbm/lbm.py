@@ -23,8 +23,9 @@ Differences, by design:
     ``Solver.f`` starts at ``q * qstride``, not ``q * n`` (the kernels take the stride); and,
     when its rows are a multiple of 64 cells, keeps its populations in tiles of a few rows
     (``default_tile_rows``; ``lbm_step``/``lbm_init`` take ``tile_rows``): each tile holds its
-    19 planes together, which streams faster than planes a lattice apart.  ``populations()``
-    returns the q-major planes in either layout;
+    19 planes together (and, with ``heat=True``, each temperature tile its 7), which streams
+    faster than planes a lattice apart.  ``populations()`` and ``temperature_populations()``
+    return the q-major planes in either layout;
   * ``LidY`` adds a moving lid; obstacle centre, sponge start, Cs/magic and a seeded velocity
     noise can be overridden;
   * with ``heat=True`` the temperature field follows the report fields: ``heat_fields["temp"]``
@@ -576,32 +577,34 @@ def default_tile_rows(nx, ny, nz, storage="f32"):
     return k
 
 
-def population_elems(lat, qstride=None, tile_rows=0):
-    """Elements of one population buffer: 19 planes of ``qstride`` (default n), or for tiles
-    of k = ``tile_rows`` rows, 19 planes of k nx cells per tile (the last tile full size)."""
+def population_elems(lat, qstride=None, tile_rows=0, q=19):
+    """Elements of one population buffer of ``q`` planes (19, or 7 for the temperature): q planes
+    of ``qstride`` (default n), or for tiles of k = ``tile_rows`` rows, q planes of k nx cells per
+    tile (the last tile full size)."""
     if tile_rows:
         k = int(tile_rows)
-        return 19 * k * lat.nx * -(-(lat.ny * lat.nz) // k)
-    return 19 * (lat.n if qstride is None else int(qstride))
+        return q * k * lat.nx * -(-(lat.ny * lat.nz) // k)
+    return q * (lat.n if qstride is None else int(qstride))
 
 
-def untile_populations(raw, lat, tile_rows):
-    """A tiled population buffer (flat, ``population_elems`` long) as q-major planes (19, n)."""
+def untile_populations(raw, lat, tile_rows, q=19):
+    """A tiled buffer of ``q`` planes (flat, ``population_elems`` long) as q-major planes (q, n)."""
     k = int(tile_rows)
     tiles = -(-(lat.ny * lat.nz) // k)
-    a = np.asarray(raw)[:19 * k * lat.nx * tiles].reshape(tiles, 19, k * lat.nx)
-    return a.transpose(1, 0, 2).reshape(19, tiles * k * lat.nx)[:, :lat.n]
+    a = np.asarray(raw)[:q * k * lat.nx * tiles].reshape(tiles, q, k * lat.nx)
+    return a.transpose(1, 0, 2).reshape(q, tiles * k * lat.nx)[:, :lat.n]
 
 
 def tile_populations(planes, lat, tile_rows):
-    """q-major planes (19, n) as a flat tiled population buffer (inverse of
-    ``untile_populations``; the rows past the lattice in the last tile are zero)."""
+    """q-major planes (q, n) as a flat tiled buffer (inverse of ``untile_populations``; the rows
+    past the lattice in the last tile are zero)."""
     k = int(tile_rows)
     tiles = -(-(lat.ny * lat.nz) // k)
     planes = np.asarray(planes)
-    out = np.zeros((19, tiles * k * lat.nx), planes.dtype)
-    out[:, :lat.n] = planes.reshape(19, lat.n)
-    return out.reshape(19, tiles, k * lat.nx).transpose(1, 0, 2).reshape(-1)
+    q = planes.shape[0]
+    out = np.zeros((q, tiles * k * lat.nx), planes.dtype)
+    out[:, :lat.n] = planes.reshape(q, lat.n)
+    return out.reshape(q, tiles, k * lat.nx).transpose(1, 0, 2).reshape(-1)
 
 
 def _check_tiled(lat, f, tile_rows):
@@ -652,8 +655,10 @@ def lbm_step(lat, phys, geo, flow, diag, f_in, f_out, heat=None, g_in=None, g_ou
     to ``g_out`` too (FP32), and a report step writes ``heat.temp`` and adds to
     ``heat.q_solid``.  ``block_x``: threads per block of this launch (0: the specialisation's
     default).  ``tile_rows`` > 0: the population buffers hold tiles of that many rows
-    (``population_elems``); the temperature planes stay q-major, their stride the g buffers'
-    length / 7.  ``xcd_group``: a quiet step's chunk order (0: the library's choice).
+    (``population_elems``), and so do the temperature buffers (7 planes per tile,
+    ``population_elems(lat, None, tile_rows, 7)``); otherwise the temperature planes share the
+    populations' plane stride.  ``xcd_group``: a quiet step's chunk order (0: the library's
+    choice).
     """
     if f_in.storage != f_out.storage:
         raise ValueError(f"f_in is {f_in.storage}, f_out is {f_out.storage}")
@@ -665,7 +670,7 @@ def lbm_step(lat, phys, geo, flow, diag, f_in, f_out, heat=None, g_in=None, g_ou
     if tile_rows:
         _check_tiled(lat, f_in, tile_rows)
         _check_tiled(lat, f_out, tile_rows)
-        qs = g_in.size // 7 if hot and g_in is not None else lat.n
+        qs = lat.n
     else:
         qs = _qstride(lat, f_in)
         if _qstride(lat, f_out) != qs:
@@ -681,9 +686,10 @@ def lbm_step(lat, phys, geo, flow, diag, f_in, f_out, heat=None, g_in=None, g_ou
         return
     if g_in is None or g_out is None:
         raise ValueError("the temperature variant needs g_in and g_out")
+    need = population_elems(lat, qs, tile_rows, 7)
     for g in (g_in, g_out):
-        if g.storage != "f32" or g.size < 7 * qs:
-            raise ValueError(f"g buffers hold 7 FP32 planes of {qs} (got {g.size} {g.storage})")
+        if g.storage != "f32" or g.size < need:
+            raise ValueError(f"g buffers need {need} FP32 elements (got {g.size} {g.storage})")
     ht = _capi.Heat(_ptr(heat.t_solid), _ptr(heat.q_solid), _ptr(heat.temp), heat.alpha,
                     heat.nu, heat.pr_t)
     check(lib().tlbm_step_heat(h, f_in.ptr, f_out.ptr, g_in.ptr, g_out.ptr, ctypes.byref(p),
@@ -805,9 +811,14 @@ class Solver:
         # q_solid takes float atomic adds from the kernel: device memory in every layout
         self.heat_fields = {"t_solid": ManagedArray(small), "q_solid": DeviceField(small, stream),
                             "temp": temp}
-        # the temperature planes share the populations' plane stride
-        planes = getattr(self, "qstride", n) if self.heat_enabled else 1
-        self.g = [zeros((7 * planes,)), zeros((7 * planes,))]
+        # the temperature planes take the populations' layout: their plane stride, or their tiles
+        # of rows with 7 planes per tile
+        tiles = getattr(self, "tile_rows", 0) if self.heat_enabled else 0
+        if tiles:
+            elems = population_elems(self.lat, None, tiles, 7)
+        else:
+            elems = 7 * (getattr(self, "qstride", n) if self.heat_enabled else 1)
+        self.g = [zeros((elems,)), zeros((elems,))]
         self._make_heat()
 
     def _make_heat(self):
@@ -1014,3 +1025,12 @@ class Solver:
         else:
             raw = self.f[0].numpy().reshape(19, self.qstride)[:, :self.lat.n]
         return decode_populations(raw, self.storage)
+
+    def temperature_populations(self):
+        """The current D3Q7 temperature populations (heat=True) as float32, shaped (7, n)."""
+        if not self.heat_enabled:
+            raise ValueError("this solver has no temperature populations (heat=False)")
+        sync(self.stream)
+        if self.tile_rows:
+            return untile_populations(self.g[0].numpy(), self.lat, self.tile_rows, 7)
+        return self.g[0].numpy().reshape(7, self.qstride)[:, :self.lat.n].copy()

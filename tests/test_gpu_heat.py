@@ -159,7 +159,7 @@ def test_solver_heat_with_padded_planes():
         s.step(report=True)
         s.update_output()
         tlbm.sync()
-        g = s.g[0].numpy().reshape(7, s.qstride)[:, :s.lat.n]
+        g = s.temperature_populations()
         out.append((s.populations(), g, s.output["temp"].numpy.copy(), s.q_solid()))
     for a, b in zip(out[0][:3], out[1][:3]):
         assert np.array_equal(a, b)

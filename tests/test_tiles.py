@@ -22,26 +22,28 @@ def _params(dims, qstride, tile_rows):
     return p
 
 
+@pytest.mark.parametrize("nq", [19, 7])
 @pytest.mark.parametrize("dims,k", [((8, 3, 4), 1), ((8, 3, 4), 4), ((8, 3, 4), 8),
                                     ((5, 7, 3), 2), ((64, 6, 5), 16)])
-def test_tile_index_formula_and_round_trip(dims, k):
-    """Population q of cell (x, r = z ny + y) sits at (r // k) 19 k nx + q k nx + (r % k) nx + x,
-    checked element by element against an independent loop; untile inverts tile."""
+def test_tile_index_formula_and_round_trip(dims, k, nq):
+    """Population q of cell (x, r = z ny + y) sits at (r // k) Q k nx + q k nx + (r % k) nx + x
+    for a buffer of Q planes (19 populations, or the 7 temperature planes), checked element by
+    element against an independent loop; untile inverts tile."""
     import tlbm
     lat = _lat(*dims)
     nx, ny, nz = dims
     rng = np.random.default_rng(7)
-    planes = rng.standard_normal((19, lat.n)).astype(np.float32)
+    planes = rng.standard_normal((nq, lat.n)).astype(np.float32)
     flat = tlbm.tile_populations(planes, lat, k)
-    assert flat.size == tlbm.population_elems(lat, None, k)
-    for q in range(19):
+    assert flat.size == tlbm.population_elems(lat, None, k, nq)
+    for q in range(nq):
         for z in range(nz):
             for y in range(ny):
                 r = z * ny + y
                 for x in range(nx):
-                    addr = (r // k) * 19 * k * nx + q * k * nx + (r % k) * nx + x
+                    addr = (r // k) * nq * k * nx + q * k * nx + (r % k) * nx + x
                     assert flat[addr] == planes[q, r * nx + x]
-    assert np.array_equal(tlbm.untile_populations(flat, lat, k), planes)
+    assert np.array_equal(tlbm.untile_populations(flat, lat, k, nq), planes)
 
 
 def test_population_elems_matches_the_library():
