@@ -132,15 +132,16 @@ __device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t voff
   }
 }
 
-// NT: non-temporal, for 4-byte elements at device scope (the "nt" and "sc1" bits, 2 | 16 in the
-// cache-policy operand), for 2-byte elements "nt" alone.  Measured at 512x256x256 (19 planes,
-// tiles of rows): the FP32 step's store pattern 0.817 ms with nt + sc1 against 0.831 with nt
-// alone and 0.909 with sc1 alone (the FP32 step 0.829 -> 0.814 ms); the FP16 step unchanged
-// and the FP16 batched sweep 2% slower with nt + sc1.
-template <class T, bool NT>
+// NT: non-temporal at device scope (the "nt" and "sc1" bits, 2 | 16 in the cache-policy
+// operand); for 2-byte elements only in the per-wave row build (ROWS), "nt" alone otherwise.
+// Measured at 512x256x256 (19 planes, tiles of rows): the FP32 step's store pattern 0.817 ms
+// with nt + sc1 against 0.831 with nt alone and 0.909 with sc1 alone (the FP32 step 0.829 ->
+// 0.814 ms); the FP16 step 0.4072-0.4092 against 0.4101-0.4110 ms, the FP16 batched sweep
+// (per-lane rows) 2% slower with nt + sc1.
+template <class T, bool NT, bool ROWS = true>
 __device__ __forceinline__ void buffer_store(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff,
                                              uint32_t soff) {
-  constexpr int pol = NT ? (sizeof(T) == 4 ? (2 | 16) : 2) : 0;
+  constexpr int pol = NT ? ((sizeof(T) == 4 || ROWS) ? (2 | 16) : 2) : 0;
   if constexpr (sizeof(T) == 4) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r,
                                           static_cast<int>(voff), static_cast<int>(soff), pol);
@@ -731,7 +732,7 @@ __device__ __forceinline__ void store_raw(const StepArgs& a, T* __restrict__ fou
     const uint32_t vo = static_cast<uint32_t>(row + x) * B;
     static_for<0, kQ>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      buffer_store<T, NT>(raw[q], rout, vo, a.plane_off[q]);
+      buffer_store<T, NT, ROWS>(raw[q], rout, vo, a.plane_off[q]);
     });
   } else {
     const uint64_t own = row + x;
