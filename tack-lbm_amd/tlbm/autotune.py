@@ -116,6 +116,25 @@ def tune_batch(batch, reps=5, use_cache=True, candidates=None):
     return best[:3]
 
 
+def tune_batch_layout(make_batch, candidates=(0, "auto"), reps=5, use_cache=True):
+    """Pick a batch's population layout as well as its launch shape: ``make_batch(tile_rows)``
+    builds the BatchSolver for one layout candidate (``BatchSolver(..., tile_rows=...)``: 0 for
+    padded q-major planes, "auto", a power of two, or one value per domain); each candidate's
+    launch shape is tuned with ``tune_batch`` and the fastest batch is kept, the others closed.
+    Returns (batch, candidate, (block, grid, ms))."""
+    best = None
+    for cand in candidates:
+        batch = make_batch(cand)
+        shape = tune_batch(batch, reps=reps, use_cache=use_cache)
+        if best is None or shape[2] < best[2][2]:
+            if best is not None:
+                best[0].close()
+            best = (batch, cand, shape)
+        else:
+            batch.close()
+    return best  # tune_batch left the kept batch with its best launch shape and chunk order
+
+
 def tune_solver(solver, reps=5, use_cache=True, blocks=(64, 128, 192, 256), groups=(1, 2, 4)):
     """Pick threads per block and the chunk order (``xcd_group``) of a Solver's quiet kernel
     (the report kernel takes the same block size); applies them with ``solver.set_launch`` and

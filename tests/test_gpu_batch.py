@@ -197,6 +197,35 @@ def test_batch_autotune_keeps_state(tmp_path, monkeypatch):
     assert again[:2] == (block, grid)  # from the cache
 
 
+def test_batch_layout_autotune(tmp_path, monkeypatch):
+    """The layout tuner builds one batch per tile-rows candidate, keeps the fastest (tuned) one
+    and closes the others; the kept batch steps as a fresh batch of its layout with its tuned
+    launch shape."""
+    import tlbm
+    from tlbm import autotune
+    from tlbm.batch import BatchSolver
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "tune.json"))
+    built = []
+
+    def make(tile_rows):
+        b = BatchSolver(_domains(), collision="trt", storage="f32", tile_rows=tile_rows)
+        built.append(b)
+        return b
+    batch, cand, (block, grid, ms) = autotune.tune_batch_layout(make, (0, 4, "auto"), reps=2)
+    assert cand in (0, 4, "auto") and ms > 0 and len(built) == 3
+    assert all(b._batch is None for b in built if b is not batch)   # the others are closed
+    ref = make(cand)
+    ref.set_launch(block, grid)
+    ref.set_order(batch.group)
+    for b in (batch, ref):
+        _random_state(b, seed=8)
+        b.step()
+        b.step(report=True)
+    tlbm.sync()
+    assert batch.tile_rows == ref.tile_rows
+    assert np.array_equal(batch.f[0].numpy(), ref.f[0].numpy())
+
+
 def test_sweep_domains_are_seeded_and_in_range():
     from tlbm.batch import sweep_domains
     a = sweep_domains(64, seed=3)
